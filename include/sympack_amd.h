@@ -1,0 +1,152 @@
+/*
+ * C ABI of the MI355X-native supernodal Cholesky / static LDL^T path.
+ *
+ * Every entry point returns an int status: 0 on success, < 0 for an argument
+ * or device error (spk_last_error() holds the message), and for the
+ * factorization > 0 = first failing pivot column (1-based, in the permuted
+ * order; LAPACK "info" semantics) where symPACK throws
+ * "Matrix is not HPD." (src/lapack.cpp:293-306).
+ *
+ * The life cycle mirrors symPACKMatrix2D (include/sympack/symPACKMatrix2D.hpp
+ * :2612-2627, driver/run_sympack2D.cpp:74-128):
+ *   spk_create -> spk_init -> spk_symbolic_factorization ->
+ *   spk_distribute_matrix -> spk_factorize -> spk_solve -> spk_get_solution
+ *   -> spk_destroy
+ * Matrices are given as the lower triangle (diagonal included) in compressed
+ * sparse column form, as DistSparseMatrix<T> holds them on one rank
+ * (include/sympack/DistSparseMatrix.hpp:46-61).  Right-hand sides and
+ * solutions are column-major n x nrhs in the caller's numbering.
+ */
+#ifndef SYMPACK_AMD_H_
+#define SYMPACK_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ordering codes: numeric values of symPACK::OrderingType
+ * (include/sympack/Types.hpp:101). */
+#define SPK_ORDER_NATURAL 0
+#define SPK_ORDER_RCM 1
+#define SPK_ORDER_MMD 2
+#define SPK_ORDER_AMD 3
+#define SPK_ORDER_NDBOX 4
+#define SPK_ORDER_NDGRID 5
+#define SPK_ORDER_USER 10
+
+/* symPACK::DecompositionType (include/sympack/Types.hpp:97). */
+#define SPK_FACT_LL 0
+#define SPK_FACT_LDL 1
+
+/* Arithmetic of the factor. */
+#define SPK_PREC_FP64 0        /* FP64 factor and solve (the drop-in) */
+#define SPK_PREC_FP32_IR 1     /* FP32 factor, FP64 iterative refinement */
+
+typedef struct spk_matrix_s* spk_handle;
+
+/* Options: the fields of symPACKOptions that run_sympack2D sets
+ * (include/sympack/Types.hpp:105-188, driver/utils.hpp:227-452). */
+typedef struct spk_options {
+  int ordering;       /* SPK_ORDER_*; default MMD */
+  int decomposition;  /* SPK_FACT_*; default LL */
+  int nrelax0, nrelax1, nrelax2; /* RelaxationParameters; default 8, 32, 64 */
+  int maxsize;        /* maximum supernode width, 0 = unlimited (driver default) */
+  double zrelax0, zrelax1, zrelax2; /* 0.8, 0.1, 0.05 */
+  int precision;      /* SPK_PREC_*; default FP64 */
+  int ir_max_iter;    /* refinement steps for SPK_PREC_FP32_IR (default 30) */
+  double ir_tol;      /* normwise backward-error target (default 1e-12) */
+  const int* perm;    /* SPK_ORDER_USER: perm[new] = old, any base (symPACKOptions.perm) */
+  int verbose;
+  int profile;        /* time every kernel launch with HIP events */
+  int device;         /* HIP device ordinal */
+} spk_options;
+
+typedef struct spk_stats {
+  int32_t n, nsuper, nlevels, max_width;
+  double flops;           /* sum cc_j^2 over unrelaxed column counts (the "Flops:" line) */
+  int64_t nnz_l;          /* sum cc_j (the "NNZ in L factor:" line) */
+  int64_t factor_entries; /* stored factor entries, relaxed supernodes incl. upper diag blocks */
+  double schur_flops;     /* sum 2 m n k of the Schur updates */
+  double inpanel_flops, diag_flops;
+  int64_t schur_entries;  /* scattered update entries */
+  int64_t launches_factor, launches_solve;
+  double factor_ms, solve_ms;     /* wall time of the last Factorize / Solve */
+  int ir_iterations;              /* last Solve */
+  double backward_error;          /* last Solve (FP32+IR), -1 if not computed */
+} spk_stats;
+
+/* Per-kernel-kind event timings (profile mode).  kind: 0 diagonal factor,
+ * 7 panel TRSM, 1 in-panel update, 2 Schur update, 3 forward diagonal,
+ * 4 forward off-diagonal, 5 backward off-diagonal, 6 backward diagonal. */
+int spk_get_kernel_stats(spk_handle h, int kind, int64_t* launches, double* ms);
+int spk_reset_kernel_stats(spk_handle h);
+
+/* Library set-up, as the reference's C entry points (include/sympack.hpp:31-36,
+ * src/global.cpp:114-185).  One rank: symPACK_Rank always returns 0. */
+int symPACK_Init(int* argc, char*** argv);
+int symPACK_Finalize(void);
+int symPACK_Rank(int* rank);
+/* Replaces symPACK_cuda_setup (src/global.cpp:32-60): select the HIP device. */
+int symPACK_device_setup(int device);
+
+void spk_default_options(spk_options* opt);
+/* Parse run_sympack2D flags (-ordering, -fact, -relax a b c, -b, -v,
+ * -precision, -ir_tol, -ir_iter) into opt (driver/utils.hpp:227-452).
+ * Unknown flags are ignored, as the reference's option map does. */
+int spk_options_from_args(spk_options* opt, int argc, const char* const* argv);
+
+int spk_create(spk_handle* h);
+int spk_destroy(spk_handle h);
+/* symPACKMatrix2D::Init (symPACKMatrix2D.hpp:2649-2695) */
+int spk_init(spk_handle h, const spk_options* opt);
+/* symPACKMatrix2D::SymbolicFactorization (symPACKMatrix2D.hpp:2699-6334):
+ * lower-triangular CSC pattern with base `baseval` (0 or 1). */
+int spk_symbolic_factorization(spk_handle h, int32_t n, const int64_t* colptr,
+                               const int32_t* rowind, int baseval);
+/* symPACKMatrix2D::DistributeMatrix (symPACKMatrix2D.hpp:6337-6576):
+ * values of the same CSC, host memory / device memory. */
+int spk_distribute_matrix(spk_handle h, const double* nzval);
+int spk_distribute_matrix_device(spk_handle h, const double* d_nzval);
+/* symPACKMatrix2D::Factorize (symPACKMatrix2D.hpp:6579-6639). */
+int spk_factorize(spk_handle h);
+/* symPACKMatrix2D::Solve (symPACKMatrix2D.hpp:6643-6710): RHS is borrowed and
+ * not modified; rhs_size is unused, as in the reference's 2D path. */
+int spk_solve(spk_handle h, const double* rhs, int nrhs, int64_t rhs_size);
+int spk_solve_device(spk_handle h, const double* d_rhs, int nrhs);
+/* symPACKMatrix2D::GetSolution (symPACKMatrix2D.hpp:6713-6757). */
+int spk_get_solution(spk_handle h, double* B, int nrhs);
+int spk_get_solution_device(spk_handle h, double* d_B, int nrhs);
+
+/* Accessors (symPACKMatrixBase.hpp:56-103): all arrays 1-based as in symPACK. */
+int spk_get_stats(spk_handle h, spk_stats* st);
+int spk_get_supernodal_partition(spk_handle h, int32_t* xsuper /* nsuper+1 */);
+int spk_get_sup_membership(spk_handle h, int32_t* supmem /* n */);
+int spk_get_ordering(spk_handle h, int32_t* perm /* n */, int32_t* invp /* n */);
+int spk_get_etree(spk_handle h, int32_t* parent /* n, postordered */);
+int spk_get_column_counts(spk_handle h, int32_t* cc_unrelaxed, int32_t* cc_relaxed);
+int spk_get_structure_size(spk_handle h, int64_t* nnz_lindx);
+int spk_get_structure(spk_handle h, int64_t* xlindx /* nsuper+1 */, int32_t* lindx);
+/* 2D cell map (symPACKMatrix2D.hpp:3317-3396): sizes, then arrays. */
+int spk_get_cell_counts(spk_handle h, int64_t* ncells, int64_t* nblocks);
+int spk_get_cells(spk_handle h, int32_t* cell_K, int32_t* cell_I, int32_t* cell_nrows,
+                  int32_t* cell_nblocks, int32_t* block_first_row, int32_t* block_nrows);
+/* Factor panel of supernode I (1-based): r x w column-major, structure rows
+ * by supernode columns; LDL panels hold D on the diagonal. */
+int spk_get_panel_dims(spk_handle h, int32_t I, int32_t* nrows, int32_t* ncols);
+int spk_get_factor_panel(spk_handle h, int32_t I, double* out);
+
+/* Rutherford-Boeing / Harwell-Boeing I/O (ReadHB_PARA, utility.hpp:1274-1482). */
+int spk_read_rb(const char* filename, int32_t* n, int64_t* nnz);
+int spk_read_rb_data(const char* filename, int64_t* colptr, int32_t* rowind, double* nzval);
+int spk_write_rb(const char* filename, int32_t n, const int64_t* colptr, const int32_t* rowind,
+                 const double* nzval, const char* title);
+
+const char* spk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMPACK_AMD_H_ */

@@ -1,0 +1,492 @@
+// Engine implementation: plan upload, factorization / solve launch sequences
+// and FP64 iterative refinement around an FP32 factor.
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <stdexcept>
+
+#include "kernels.hip"
+
+namespace spk {
+
+std::string hip_error_string(hipError_t e) { return std::string(hipGetErrorString(e)); }
+
+#define HIP_OK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__) + " (" #expr ")");  \
+  } while (0)
+
+namespace {
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIP_OK(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+template <typename T>
+T* dupload(const std::vector<T>& v, hipStream_t s) {
+  T* p = dalloc<T>(v.size());
+  if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return p;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+inline unsigned grid_for(int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
+}
+enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF };
+}  // namespace
+
+Engine::Engine() { kind_stats.resize(8); }
+
+Engine::~Engine() {
+  try {
+    free_device();
+    for (auto& e : ev_pool_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  } catch (...) {
+  }
+}
+
+void Engine::free_device() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  dfree(d_sn_);
+  dfree(d_lindx0_);
+  dfree(d_tcol_);
+  dfree(d_tmap_);
+  dfree(d_rel_);
+  dfree(d_amap_);
+  dfree(d_perm0_);
+  for (auto& t : d_tasks_) dfree(t);
+  dfree(d_info_);
+  dfree(d_acolptr_);
+  dfree(d_arowind_);
+  dfree(d_acolidx_);
+  dfree(d_vals_);
+  dfree(d_scale_);
+  dfree(d_arena64_);
+  dfree(d_arena32_);
+  dfree(d_Y_);
+  dfree(d_X_);
+  dfree(d_b_);
+  dfree(d_x_);
+  dfree(d_r_);
+  dfree(d_red_);
+  cap_nrhs_ = 0;
+}
+
+void Engine::init(const EngineOptions& o) {
+  // no device work here: the host analysis runs without a GPU
+  opt_ = o;
+}
+
+void Engine::reset_stats() {
+  for (auto& k : kind_stats) k = KindStats();
+}
+
+void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) {
+  free_device();
+  have_symbolic_ = have_values_ = factored_ = device_ready_ = false;
+  const Int* up = opt_.user_perm.empty() ? nullptr : opt_.user_perm.data();
+  symbolic_factorization(n, colptr, rowind, baseval, opt_.ordering, opt_.relax, up, sym_);
+  build_plan(sym_, n, colptr, rowind, baseval, plan_);
+  n_ = n;
+  nnzA_ = colptr[n] - baseval;
+  h_colptr_.assign(n + 1, 0);
+  for (Int i = 0; i <= n; ++i) h_colptr_[i] = colptr[i] - baseval;
+  h_rowind_.assign(nnzA_, 0);
+  for (Int c = 0; c < n; ++c)
+    for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) h_rowind_[p] = rowind[p] - baseval;
+  have_symbolic_ = true;
+}
+
+// Device set-up on first numeric use: stream, plan, maps, task lists, arena.
+void Engine::ensure_device() {
+  if (!stream_) {
+    HIP_OK(hipSetDevice(opt_.device));
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  }
+  if (device_ready_) return;
+  hipStream_t s = stream_;
+  std::vector<int32_t> colidx(nnzA_);
+  for (Int c = 0; c < n_; ++c)
+    for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) colidx[p] = c;
+  d_sn_ = dupload(plan_.sn, s);
+  d_lindx0_ = dupload(plan_.lindx0, s);
+  d_tcol_ = dupload(plan_.tcol, s);
+  d_tmap_ = dupload(plan_.tmap, s);
+  d_rel_ = dupload(plan_.rel, s);
+  d_amap_ = dupload(plan_.amap, s);
+  d_perm0_ = dupload(plan_.perm0, s);
+  d_tasks_[TA_POTRF] = dupload(plan_.potrf_tasks, s);
+  d_tasks_[TA_TRSM] = dupload(plan_.trsm_tasks, s);
+  d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
+  d_tasks_[TA_SCHUR] = dupload(plan_.schur_tasks, s);
+  d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
+  d_tasks_[TA_SOFF] = dupload(plan_.soff_tasks, s);
+  d_info_ = dalloc<int>(1);
+  d_acolptr_ = dupload(h_colptr_, s);
+  d_arowind_ = dupload(h_rowind_, s);
+  d_acolidx_ = dupload(colidx, s);
+  d_vals_ = dalloc<double>(nnzA_);
+  d_scale_ = dalloc<double>(n_);
+  if (opt_.precision == 0)
+    d_arena64_ = dalloc<double>(plan_.arena_elems);
+  else
+    d_arena32_ = dalloc<float>(plan_.arena_elems);
+  HIP_OK(hipStreamSynchronize(s));
+  device_ready_ = true;
+}
+
+void Engine::distribute(const double* nzval, bool on_device) {
+  if (!have_symbolic_) throw std::logic_error("DistributeMatrix called before SymbolicFactorization");
+  ensure_device();
+  HIP_OK(hipMemcpyAsync(d_vals_, nzval, nnzA_ * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  // ||A||_inf of the symmetric matrix and the equilibration scaling
+  // s_i = 1/sqrt(|a_ii|) used by the FP32 factor.
+  std::vector<double> vals;
+  const double* hv = nzval;
+  if (on_device) {
+    vals.resize(nnzA_);
+    HIP_OK(hipMemcpyAsync(vals.data(), nzval, nnzA_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    hv = vals.data();
+  }
+  std::vector<double> rowsum(n_, 0.0), diag(n_, 0.0);
+  for (Int c = 0; c < n_; ++c)
+    for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) {
+      int32_t r = h_rowind_[p];
+      double a = std::fabs(hv[p]);
+      rowsum[r] += a;
+      if (r != c)
+        rowsum[c] += a;
+      else
+        diag[c] = hv[p];
+    }
+  normA_inf_ = 0.0;
+  for (double v : rowsum) normA_inf_ = std::max(normA_inf_, v);
+  std::vector<double> sc(n_, 1.0);
+  if (opt_.precision == 1)
+    for (Int i = 0; i < n_; ++i) sc[i] = diag[i] != 0.0 ? 1.0 / std::sqrt(std::fabs(diag[i])) : 1.0;
+  HIP_OK(hipMemcpyAsync(d_scale_, sc.data(), n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+  have_values_ = true;
+  factored_ = false;
+}
+
+// Equilibrated assembly: entry (r,c) scaled by s_r s_c.
+template <typename T>
+__global__ void k_assemble_scaled(T* __restrict__ arena, const int64_t* __restrict__ amap,
+                                  const double* __restrict__ vals, const int32_t* __restrict__ rowind,
+                                  const int32_t* __restrict__ colidx, const double* __restrict__ s,
+                                  int64_t nnz) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t dst = amap[e];
+    if (dst < 0) continue;
+    arena[dst] = (T)(vals[e] * s[rowind[e]] * s[colidx[e]]);
+  }
+}
+
+template <typename T>
+void Engine::assemble(T* arena) {
+  dev::k_fill_zero<T><<<grid_for(plan_.arena_elems), 256, 0, stream_>>>(arena, plan_.arena_elems);
+  if (opt_.precision == 0)
+    dev::k_assemble<T><<<grid_for(nnzA_), 256, 0, stream_>>>(arena, d_amap_, d_vals_, nullptr, nnzA_);
+  else
+    k_assemble_scaled<T><<<grid_for(nnzA_), 256, 0, stream_>>>(arena, d_amap_, d_vals_, d_arowind_,
+                                                               d_acolidx_, d_scale_, nnzA_);
+  HIP_OK(hipGetLastError());
+}
+
+void Engine::launch_begin(int kind) {
+  kind_stats[kind].launches++;
+  if (!opt_.profile) return;
+  while (ev_pool_.size() < ev_next_ + 2) {
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
+  hipEvent_t a = ev_pool_[ev_next_], b = ev_pool_[ev_next_ + 1];
+  ev_next_ += 2;
+  HIP_OK(hipEventRecord(a, stream_));
+  ev_open_.push_back({kind, {a, b}});
+}
+
+void Engine::launch_end(int kind) {
+  (void)kind;
+  if (!opt_.profile) return;
+  HIP_OK(hipEventRecord(ev_open_.back().second.second, stream_));
+}
+
+void Engine::collect_events() {
+  if (!opt_.profile) return;
+  for (auto& o : ev_open_) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, o.second.first, o.second.second));
+    kind_stats[o.first].ms += ms;
+  }
+  ev_open_.clear();
+  ev_next_ = 0;
+}
+
+template <typename T>
+void Engine::factor_sequence(T* arena) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const bool ldl = opt_.ldl != 0;
+  for (const LaunchRange& r : plan_.factor_seq) {
+    const unsigned g = (unsigned)r.count;
+    launch_begin(r.kind);
+    switch (r.kind) {
+      case 0: {
+        const Task4* t = (const Task4*)d_tasks_[TA_POTRF] + r.first;
+        if (ldl)
+          dev::k_potrf<T, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_info_);
+        else
+          dev::k_potrf<T, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_info_);
+        break;
+      }
+      case 7: {
+        const Task4* t = (const Task4*)d_tasks_[TA_TRSM] + r.first;
+        if (ldl)
+          dev::k_trsm<T, true><<<g, 256, 0, stream_>>>(sn, t, arena);
+        else
+          dev::k_trsm<T, false><<<g, 256, 0, stream_>>>(sn, t, arena);
+        break;
+      }
+      case 1: {
+        const Task4* t = (const Task4*)d_tasks_[TA_INPANEL] + r.first;
+        if (ldl)
+          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+        else
+          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+        break;
+      }
+      case 2: {
+        const Task4* t = (const Task4*)d_tasks_[TA_SCHUR] + r.first;
+        if (ldl)
+          dev::k_update<T, 1, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+        else
+          dev::k_update<T, 1, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+        break;
+      }
+      default:
+        throw std::logic_error("bad factor launch kind");
+    }
+    launch_end(r.kind);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+int Engine::factorize() {
+  if (!have_values_) throw std::logic_error("Factorize called before DistributeMatrix");
+  auto t0 = std::chrono::steady_clock::now();
+  const int big = INT_MAX;
+  HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
+  if (opt_.precision == 0) {
+    assemble(d_arena64_);
+    factor_sequence(d_arena64_);
+  } else {
+    assemble(d_arena32_);
+    factor_sequence(d_arena32_);
+  }
+  int info = 0;
+  HIP_OK(hipMemcpyAsync(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+  collect_events();
+  last_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  factored_ = (info == big);
+  return factored_ ? 0 : info;
+}
+
+void Engine::ensure_solve_capacity(int nrhs) {
+  if (nrhs <= cap_nrhs_) return;
+  dfree(d_Y_);
+  dfree(d_X_);
+  dfree(d_b_);
+  dfree(d_x_);
+  dfree(d_r_);
+  dfree(d_red_);
+  size_t tsz = opt_.precision == 0 ? sizeof(double) : sizeof(float);
+  size_t cnt = (size_t)n_ * nrhs;
+  HIP_OK(hipMalloc(&d_Y_, std::max<size_t>(1, cnt) * tsz));
+  HIP_OK(hipMalloc(&d_X_, std::max<size_t>(1, cnt) * tsz));
+  d_b_ = dalloc<double>(cnt);
+  d_x_ = dalloc<double>(cnt);
+  d_r_ = dalloc<double>(cnt);
+  d_red_ = dalloc<double>(2 * (size_t)nrhs);
+  cap_nrhs_ = nrhs;
+}
+
+template <typename T>
+void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const bool ldl = opt_.ldl != 0;
+  const Task4* sd = (const Task4*)d_tasks_[TA_SDIAG];
+  const Task4* so = (const Task4*)d_tasks_[TA_SOFF];
+  for (const LaunchRange& r : plan_.fwd_seq) {
+    const unsigned g = (unsigned)r.count;
+    launch_begin(r.kind);
+    if (r.kind == 3) {
+      if (ldl)
+        dev::k_sdiag_fwd<T, true><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
+      else
+        dev::k_sdiag_fwd<T, false><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
+    } else {
+      dev::k_soff_fwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, Y, nrhs);
+    }
+    launch_end(r.kind);
+  }
+  if (ldl) {
+    launch_begin(3);
+    dev::k_scale_d<T><<<(unsigned)plan_.nsuper, 256, 0, stream_>>>(sn, sd, arena, X, nrhs);
+    launch_end(3);
+  }
+  for (const LaunchRange& r : plan_.bwd_seq) {
+    const unsigned g = (unsigned)r.count;
+    launch_begin(r.kind);
+    if (r.kind == 5) {
+      dev::k_soff_bwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, nrhs);
+    } else {
+      if (ldl)
+        dev::k_sdiag_bwd<T, true><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, X, nrhs);
+      else
+        dev::k_sdiag_bwd<T, false><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, X, nrhs);
+    }
+    launch_end(r.kind);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+// r = b - A x (FP64, original numbering).
+void Engine::residual(const double* b, const double* x, double* r, int nrhs) {
+  HIP_OK(hipMemcpyAsync(r, b, (size_t)n_ * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  const int cols_per_block = 4;
+  unsigned g = (unsigned)((n_ + cols_per_block - 1) / cols_per_block);
+  dev::k_residual<<<g, 64 * cols_per_block, 0, stream_>>>(d_acolptr_, d_arowind_, d_vals_, x, r, n_, nrhs);
+  HIP_OK(hipGetLastError());
+}
+
+// per-RHS max |r| and max |x| (column-major n x nrhs), one block per column
+__global__ void k_colmax2(const double* __restrict__ r, const double* __restrict__ x, int n,
+                          double* __restrict__ out) {
+  __shared__ double sr[256], sx[256];
+  int j = blockIdx.x;
+  double mr = 0.0, mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    mr = fmax(mr, fabs(r[(int64_t)j * n + i]));
+    mx = fmax(mx, fabs(x[(int64_t)j * n + i]));
+  }
+  sr[threadIdx.x] = mr;
+  sx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + s]);
+      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * j] = sr[0];
+    out[2 * j + 1] = sx[0];
+  }
+}
+
+SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
+  if (!factored_) throw std::logic_error("Solve called without a successful Factorize");
+  if (nrhs <= 0) throw std::invalid_argument("nrhs must be positive");
+  auto t0 = std::chrono::steady_clock::now();
+  ensure_solve_capacity(nrhs);
+  const size_t cnt = (size_t)n_ * nrhs;
+  HIP_OK(hipMemcpyAsync(d_b_, rhs, cnt * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  SolveReport rep;
+  const unsigned g = grid_for((int64_t)cnt);
+  if (opt_.precision == 0) {
+    double* Y = (double*)d_Y_;
+    double* X = (double*)d_X_;
+    dev::k_gather_rhs<double><<<g, 256, 0, stream_>>>(d_b_, Y, d_perm0_, n_, nrhs, nullptr);
+    solve_sequence<double>(d_arena64_, Y, X, nrhs);
+    dev::k_scatter_sol<double><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, nullptr, 0);
+    HIP_OK(hipGetLastError());
+  } else {
+    // FP64 iterative refinement around the FP32 factor of S A S
+    // (S = diag(s)): x_0 = 0, r_0 = b; d = S (SAS)^{-1} S r; x += d;
+    // r = b - A x in FP64; stop when the normwise backward error is met.
+    float* Y = (float*)d_Y_;
+    float* X = (float*)d_X_;
+    dev::k_fill_zero<double><<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_x_, (int64_t)cnt);
+    HIP_OK(hipMemcpyAsync(d_r_, d_b_, cnt * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    // ||b||_inf per rhs (x = 0 at this point)
+    std::vector<double> red(2 * (size_t)nrhs), bnorm(nrhs);
+    k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_b_, d_x_, n_, d_red_);
+    HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (int j = 0; j < nrhs; ++j) bnorm[j] = red[2 * j];
+    rep.converged = 0;
+    for (int it = 0; it < std::max(1, opt_.ir_max_iter); ++it) {
+      dev::k_gather_rhs<float><<<g, 256, 0, stream_>>>(d_r_, Y, d_perm0_, n_, nrhs, d_scale_);
+      solve_sequence<float>(d_arena32_, Y, X, nrhs);
+      dev::k_scatter_sol<float><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
+      residual(d_b_, d_x_, d_r_, nrhs);
+      k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+      HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      double berr = 0.0;
+      for (int j = 0; j < nrhs; ++j) {
+        double den = normA_inf_ * red[2 * j + 1] + bnorm[j];
+        berr = std::max(berr, den > 0 ? red[2 * j] / den : 0.0);
+      }
+      rep.iterations = it + 1;
+      rep.backward_error = berr;
+      if (berr <= opt_.ir_tol) {
+        rep.converged = 1;
+        break;
+      }
+    }
+  }
+  HIP_OK(hipStreamSynchronize(stream_));
+  collect_events();
+  last_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  last_report = rep;
+  return rep;
+}
+
+void Engine::get_solution(double* B, int nrhs, bool on_device) {
+  if (nrhs > cap_nrhs_) throw std::logic_error("GetSolution: nrhs larger than the last Solve");
+  HIP_OK(hipMemcpyAsync(B, d_x_, (size_t)n_ * nrhs * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_panel(Int I, double* out) const {
+  if (I < 0 || I >= plan_.nsuper) throw std::out_of_range("supernode index out of range");
+  const SnInfo& s = plan_.sn[I];
+  size_t cnt = (size_t)s.r * s.w;
+  if (opt_.precision == 0) {
+    HIP_OK(hipMemcpyAsync(out, d_arena64_ + s.off, cnt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+  } else {
+    std::vector<float> tmp(cnt);
+    HIP_OK(hipMemcpyAsync(tmp.data(), d_arena32_ + s.off, cnt * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < cnt; ++i) out[i] = tmp[i];
+  }
+}
+
+}  // namespace spk

@@ -1,0 +1,126 @@
+// Device engine: owns the HBM-resident factor, the plan and the launch
+// sequences for factorization and solve on one GPU.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../analysis/analysis.hpp"
+#include "plan.hpp"
+
+namespace spk {
+
+struct EngineOptions {
+  int ordering = ORD_MMD;
+  int ldl = 0;            // 0: LL^T, 1: static LDL^T
+  RelaxParams relax;
+  int precision = 0;      // 0: FP64 factor/solve; 1: FP32 factor + FP64 refinement
+  int ir_max_iter = 30;
+  double ir_tol = 1e-12;  // normwise backward-error target of the refinement
+  int device = 0;
+  int profile = 0;        // record HIP events around every launch
+  int verbose = 0;
+  std::vector<Int> user_perm;
+};
+
+struct KindStats {
+  int64_t launches = 0;
+  double ms = 0.0;        // summed event time (profile mode)
+};
+
+struct SolveReport {
+  int iterations = 0;       // refinement steps taken (0 for plain FP64)
+  double backward_error = -1.0;  // max over RHS of ||b-Ax||inf/(||A||inf||x||inf+||b||inf)
+  int converged = 1;
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  void init(const EngineOptions& o);
+  void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
+  void symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval);
+  void distribute(const double* nzval, bool on_device);
+  int factorize();
+  SolveReport solve(const double* rhs, int nrhs, bool on_device);
+  void get_solution(double* B, int nrhs, bool on_device);
+
+  const Symbolic& sym() const { return sym_; }
+  const Plan& plan() const { return plan_; }
+  const EngineOptions& options() const { return opt_; }
+  hipStream_t stream() const { return stream_; }
+  // copy the panel of supernode I (0-based) to host, as double
+  void get_panel(Int I, double* out) const;
+  std::vector<KindStats> kind_stats;  // indexed by LaunchRange::kind
+  void reset_stats();
+  double last_factor_ms = 0.0, last_solve_ms = 0.0;
+  SolveReport last_report;
+  bool factored() const { return factored_; }
+
+ private:
+  void free_device();
+  template <typename T>
+  void factor_sequence(T* arena);
+  template <typename T>
+  void solve_sequence(const T* arena, T* Y, T* X, int nrhs);
+  template <typename T>
+  void assemble(T* arena);
+  void residual(const double* b, const double* x, double* r, int nrhs);
+  void launch_begin(int kind);
+  void launch_end(int kind);
+
+  EngineOptions opt_;
+  Symbolic sym_;
+  Plan plan_;
+  bool have_symbolic_ = false, have_values_ = false, factored_ = false, device_ready_ = false;
+  void ensure_device();
+  hipStream_t stream_ = nullptr;
+  // caller CSC (0-based) kept for assembly and residuals
+  Int n_ = 0;
+  int64_t nnzA_ = 0;
+  std::vector<int64_t> h_colptr_;
+  std::vector<int32_t> h_rowind_;
+  // device memory
+  void* d_sn_ = nullptr;
+  int32_t* d_lindx0_ = nullptr;
+  int64_t* d_tcol_ = nullptr;
+  int64_t* d_tmap_ = nullptr;
+  int32_t* d_rel_ = nullptr;
+  int64_t* d_amap_ = nullptr;
+  int32_t* d_perm0_ = nullptr;
+  void* d_tasks_[8] = {nullptr};  // potrf, trsm, inpanel, schur, sdiag, soff
+  int* d_info_ = nullptr;
+  int64_t* d_acolptr_ = nullptr;
+  int32_t* d_arowind_ = nullptr;
+  int32_t* d_acolidx_ = nullptr;
+  double* d_vals_ = nullptr;
+  double* d_scale_ = nullptr;  // equilibration s_i (FP32 mode)
+  double* d_arena64_ = nullptr;
+  float* d_arena32_ = nullptr;
+  // solve work
+  int cap_nrhs_ = 0;
+  void* d_Y_ = nullptr;
+  void* d_X_ = nullptr;
+  double* d_b_ = nullptr;
+  double* d_x_ = nullptr;
+  double* d_r_ = nullptr;
+  double* d_red_ = nullptr;
+  void ensure_solve_capacity(int nrhs);
+  double normA_inf_ = 0.0;
+  // profiling events
+  std::vector<hipEvent_t> ev_pool_;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
+  size_t ev_next_ = 0;
+  void collect_events();
+};
+
+std::string hip_error_string(hipError_t e);
+
+}  // namespace spk

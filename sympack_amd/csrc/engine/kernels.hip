@@ -1,0 +1,603 @@
+// gfx950 kernels for the supernodal LL^T / static LDL^T factorization and
+// the supernodal triangular solve.
+//
+// Factor layout: every supernode I owns a panel of r_I x w_I entries
+// (structure rows x columns), column-major with leading dimension r_I, inside
+// one HBM arena.  Rows 0..w_I-1 of the panel are the diagonal block.  In LDL
+// mode the diagonal of that block holds D and the strictly lower part the
+// unit-lower factor, as in symPACK's blockCellLDL_t
+// (include/sympack/symPACKMatrix2D.hpp:2096-2139).
+//
+// Kernels (one workgroup = 256 threads = 4 wave64):
+//  k_potrf       diagonal-block Cholesky/LDL^T in LDS (blockCell_t::factorize,
+//                symPACKMatrix2D.hpp:1052-1102).
+//  k_trsm        panel TRSM of one 64-row tile below that block
+//                (blockCell_t::trsm, :1104-1178).
+//  k_update<0>   trailing update of a wide panel inside itself (k = 64).
+//  k_update<1>   Schur update of ancestors: -P P^T (LDL: -P D P^T) on FP64 /
+//                FP32 MFMA 16x16x4, scattered straight from registers into the
+//                target panels through precomputed relative indices with
+//                hardware atomics (blockCell_t::update, :1180-1466).
+//  k_sdiag_fwd / k_soff_fwd / k_scale_d / k_soff_bwd / k_sdiag_bwd
+//                supernodal forward/backward solve (forward_update[_contrib],
+//                back_update[_contrib], :1486-1797).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace spk {
+namespace dev {
+
+constexpr int NB = kNB;
+constexpr int TILE = kTile;
+constexpr int NT = 256;
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// ---------------------------------------------------------------------------
+// Zero fill / assembly
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill_zero(T* __restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = T(0);
+}
+
+// arena[amap[e]] = scale * vals[e]; amap[e] < 0 marks a superseded duplicate.
+// With equilibration, scale = s_row * s_col per entry (esc != nullptr).
+template <typename T>
+__global__ void k_assemble(T* __restrict__ arena, const int64_t* __restrict__ amap,
+                           const double* __restrict__ vals, const double* __restrict__ esc,
+                           int64_t nnz) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t dst = amap[e];
+    if (dst < 0) continue;
+    double v = vals[e];
+    if (esc) v *= esc[e];
+    arena[dst] = (T)v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block factorization (bs <= 64) in LDS.  Task {sn, step, -, -}.
+// info receives the smallest failing pivot column (1-based, permuted order).
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) void k_potrf(const SnInfo* __restrict__ sn,
+                                               const Task4* __restrict__ tasks,
+                                               T* __restrict__ arena, int* __restrict__ info) {
+  __shared__ T Ld[NB][NB + 1];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const int c0 = t.b * NB;
+  const int bs = min(NB, s.w - c0);
+  T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  for (int idx = tid; idx < bs * bs; idx += NT) {
+    int i = idx % bs, j = idx / bs;
+    if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
+  }
+  __syncthreads();
+  for (int j = 0; j < bs; ++j) {
+    // pivot column: only wave 0 (bs - j <= 64 lanes); every lane reads the
+    // pivot in the same LDS instruction before lane 0 overwrites it.
+    if (tid < bs - j) {
+      T ajj = Ld[j][j];
+      T aij = Ld[j + tid][j];
+      bool bad = LDL ? (ajj == T(0) || ajj != ajj) : !(ajj > T(0));
+      if (bad && tid == 0) atomicMin(info, s.fc + c0 + j + 1);
+      if (!LDL) {
+        T d = bad ? T(1) : sqrt(ajj);
+        Ld[j + tid][j] = (tid == 0) ? d : aij / d;
+      } else {
+        T d = bad ? T(1) : ajj;
+        if (tid > 0) Ld[j + tid][j] = aij / d;
+      }
+    }
+    __syncthreads();
+    const int m = bs - j - 1;
+    const T dj = LDL ? Ld[j][j] : T(1);
+    for (int idx = tid; idx < m * m; idx += NT) {
+      int ii = idx % m, kk = idx / m;
+      if (ii >= kk) {
+        int i = j + 1 + ii, k = j + 1 + kk;
+        Ld[i][k] -= Ld[i][j] * Ld[k][j] * dj;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < bs * bs; idx += NT) {
+    int i = idx % bs, j = idx / bs;
+    if (i >= j) P[(int64_t)(c0 + j) * ld + c0 + i] = Ld[i][j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Panel TRSM of one 64-row tile below a factored diagonal block:
+// B <- B L^{-T} (LL) or B <- B L^{-T} D^{-1} (LDL, unit L).
+// Task {sn, step, row_tile, -}.
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
+                                              const Task4* __restrict__ tasks,
+                                              T* __restrict__ arena) {
+  __shared__ T Ld[NB][NB + 1];
+  __shared__ T X[TILE][NB + 1];
+  __shared__ T invd[NB];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const int c0 = t.b * NB;
+  const int bs = min(NB, s.w - c0);
+  T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int R0 = c0 + bs + t.c * TILE;
+  const int nr = min(TILE, s.r - R0);
+  if (nr <= 0) return;  // uniform across the workgroup
+  for (int idx = tid; idx < bs * bs; idx += NT) {
+    int i = idx % bs, j = idx / bs;
+    if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
+  }
+  for (int idx = tid; idx < TILE * bs; idx += NT) {
+    int rr = idx % TILE, j = idx / TILE;
+    X[rr][j] = (rr < nr) ? P[(int64_t)(c0 + j) * ld + R0 + rr] : T(0);
+  }
+  __syncthreads();
+  if (tid < bs) invd[tid] = T(1) / Ld[tid][tid];
+  __syncthreads();
+  // row-wise substitution, four threads per row, one barrier per column.
+  const int rr = tid % TILE, q = tid / TILE;
+  for (int j = 0; j < bs; ++j) {
+    const T xj = LDL ? X[rr][j] : X[rr][j] * invd[j];
+    for (int k = j + 1 + q; k < bs; k += 4) X[rr][k] -= xj * Ld[k][j];
+    __syncthreads();
+  }
+  for (int idx = tid; idx < TILE * bs; idx += NT) {
+    int r2 = idx % TILE, j = idx / TILE;
+    if (r2 < nr) P[(int64_t)(c0 + j) * ld + R0 + r2] = X[r2][j] * invd[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tile update.  The 64x64 output tile C[a][b] = sum_k Pa[a][k] Pb[b][k]
+// (times D[k] for LDL) is computed by 4 waves, each owning a 32x32 quarter as
+// 2x2 blocks of 16x16x4 MFMA.  MFMA operand A is the b side and operand B the
+// a side, so an accumulator element (lane l, slot v) holds
+// b = 4*(l/16)+v, a = l%16 within its block: 16 consecutive lanes own 16
+// consecutive target rows, which keeps the scatter coalesced.
+// MODE 0: in-panel trailing update of step t.b (direct read-modify-write).
+// MODE 1: Schur update of ancestor panels (atomic scatter).
+// ---------------------------------------------------------------------------
+template <typename T, int MODE, bool LDL>
+__global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
+                                                const Task4* __restrict__ tasks,
+                                                T* __restrict__ arena,
+                                                const int64_t* __restrict__ tcol,
+                                                const int64_t* __restrict__ tmap,
+                                                const int32_t* __restrict__ rel) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int KC = 16;
+  constexpr int LDS_LD = TILE + 16;
+  __shared__ T As[KC][LDS_LD];
+  __shared__ T Bs[KC][LDS_LD];
+  __shared__ T Dk[KC];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  int kbeg, K, arow0, brow0, alim, blim, ti, tj;
+  if (MODE == 0) {
+    kbeg = t.b * NB;
+    K = min(NB, s.w - kbeg);
+    const int rb = kbeg + K;
+    ti = t.c;
+    tj = t.d;
+    arow0 = rb + ti * TILE;
+    brow0 = rb + tj * TILE;
+    alim = s.r;
+    blim = s.w;
+  } else {
+    kbeg = 0;
+    K = s.w;
+    ti = t.b;
+    tj = t.c;
+    arow0 = s.w + ti * TILE;
+    brow0 = s.w + tj * TILE;
+    alim = s.r;
+    blim = s.r;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wb = wave >> 1, wa = wave & 1;
+  v4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
+
+  const int lrow = tid & 63, lk = tid >> 6;
+  const bool arow_ok = arow0 + lrow < alim;
+  const bool brow_ok = brow0 + lrow < blim;
+  for (int kc = 0; kc < K; kc += KC) {
+#pragma unroll
+    for (int q = 0; q < KC / 4; ++q) {
+      const int k = lk + 4 * q;
+      const int kg = kc + k;
+      T av = T(0), bv = T(0);
+      if (kg < K) {
+        const T* col = P + (int64_t)(kbeg + kg) * ld;
+        if (arow_ok) av = col[arow0 + lrow];
+        if (brow_ok) bv = col[brow0 + lrow];
+      }
+      As[k][lrow] = av;
+      Bs[k][lrow] = bv;
+    }
+    if (LDL && tid < KC) {
+      const int kg = kc + tid;
+      Dk[tid] = (kg < K) ? P[(int64_t)(kbeg + kg) * ld + kbeg + kg] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC / 4; ++kk) {
+      const int k = kk * 4 + (lane >> 4);
+      T bf[2], af[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        bf[bi] = Bs[k][wb * 32 + bi * 16 + (lane & 15)];
+        if (LDL) bf[bi] *= Dk[k];
+      }
+#pragma unroll
+      for (int aj = 0; aj < 2; ++aj) af[aj] = As[k][wa * 32 + aj * 16 + (lane & 15)];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) acc[bi][aj] = Mfma<T>::mma(bf[bi], af[aj], acc[bi][aj]);
+    }
+    __syncthreads();
+  }
+
+  if (MODE == 0) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int aj = 0; aj < 2; ++aj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int bg = brow0 + wb * 32 + bi * 16 + 4 * (lane >> 4) + v;
+          const int ag = arow0 + wa * 32 + aj * 16 + (lane & 15);
+          if (ag < s.r && bg < s.w && ag >= bg) P[(int64_t)bg * ld + ag] -= acc[bi][aj][v];
+        }
+  } else {
+    const int m = s.r - s.w;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int boff = tj * TILE + wb * 32 + bi * 16 + 4 * (lane >> 4) + v;
+        if (boff >= m) continue;
+        const int64_t tc = tcol[s.upd + boff];
+        const int64_t tm = tmap[s.upd + boff];
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) {
+          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
+          if (aoff < m && aoff >= boff) atomic_add(arena + tc + rel[tm + aoff], -acc[bi][aj][v]);
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solve.  Vectors are row-major n x nrhs in the permuted numbering.
+// Forward:  X_I = L_II^{-1} Y_I  (k_sdiag_fwd), then Y[struct rows] -= L_off X_I
+//           (k_soff_fwd, atomics).  LDL then scales by D^{-1} (k_scale_d).
+// Backward: X_I -= L_off^T X[struct rows] (k_soff_bwd, atomics into X_I),
+//           then X_I = L_II^{-T} X_I (k_sdiag_bwd).
+// ---------------------------------------------------------------------------
+constexpr int JC = 64;  // right-hand sides per LDS chunk
+
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) void k_sdiag_fwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   const T* __restrict__ Y, T* __restrict__ X,
+                                                   int nrhs) {
+  __shared__ T Ld[NB][NB + 1];
+  __shared__ T Z[NB][JC + 1];
+  __shared__ T invd[NB];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int64_t base = (int64_t)s.fc * nrhs;
+  for (int64_t i = tid; i < (int64_t)s.w * nrhs; i += NT) X[base + i] = Y[base + i];
+  __syncthreads();
+  for (int c0 = 0; c0 < s.w; c0 += NB) {
+    const int bs = min(NB, s.w - c0);
+    for (int idx = tid; idx < bs * bs; idx += NT) {
+      int i = idx % bs, j = idx / bs;
+      if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
+    }
+    __syncthreads();
+    if (tid < bs) invd[tid] = LDL ? T(1) : T(1) / Ld[tid][tid];
+    for (int j0 = 0; j0 < nrhs; j0 += JC) {
+      const int jc = min(JC, nrhs - j0);
+      for (int idx = tid; idx < bs * jc; idx += NT) {
+        int i = idx / jc, jj = idx % jc;
+        Z[i][jj] = X[base + (int64_t)(c0 + i) * nrhs + j0 + jj];
+      }
+      __syncthreads();
+      for (int k = 0; k < bs; ++k) {
+        const int cnt = (bs - k - 1) * jc;
+        for (int idx = tid; idx < cnt; idx += NT) {
+          int i = k + 1 + idx / jc, jj = idx % jc;
+          Z[i][jj] -= Ld[i][k] * (Z[k][jj] * invd[k]);
+        }
+        __syncthreads();
+      }
+      for (int idx = tid; idx < bs * jc; idx += NT) {
+        int i = idx / jc, jj = idx % jc;
+        T v = Z[i][jj] * invd[i];
+        X[base + (int64_t)(c0 + i) * nrhs + j0 + jj] = v;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < bs * jc; idx += NT) {
+        int i = idx / jc, jj = idx % jc;
+        Z[i][jj] *= invd[i];
+      }
+      __syncthreads();
+      // rows of this supernode below the block
+      const int below = s.w - c0 - bs;
+      for (int idx = tid; idx < below * jc; idx += NT) {
+        int i = c0 + bs + idx / jc, jj = idx % jc;
+        T acc = T(0);
+        for (int k = 0; k < bs; ++k) acc += P[(int64_t)(c0 + k) * ld + i] * Z[k][jj];
+        X[base + (int64_t)i * nrhs + j0 + jj] -= acc;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Off-diagonal tile (rows rt*64.., columns kt*64.. of the panel below the
+// diagonal block).  Task {sn, row_tile, k_tile, -}.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_soff_fwd(const SnInfo* __restrict__ sn,
+                                                  const Task4* __restrict__ tasks,
+                                                  const T* __restrict__ arena,
+                                                  const int32_t* __restrict__ lindx0,
+                                                  const T* __restrict__ X, T* __restrict__ Y,
+                                                  int nrhs) {
+  __shared__ T Ps[NB][TILE + 1];  // Ps[k][a]
+  __shared__ T Xs[NB][JC + 1];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int m = s.r - s.w;
+  const int a0 = t.b * TILE, k0 = t.c * NB;
+  const int na = min(TILE, m - a0), nk = min(NB, s.w - k0);
+  for (int idx = tid; idx < NB * TILE; idx += NT) {
+    int a = idx % TILE, k = idx / TILE;
+    Ps[k][a] = (a < na && k < nk) ? P[(int64_t)(k0 + k) * ld + s.w + a0 + a] : T(0);
+  }
+  const int a = tid % TILE, g = tid / TILE;
+  const int row = (a < na) ? lindx0[s.lx + s.w + a0 + a] : 0;
+  for (int j0 = 0; j0 < nrhs; j0 += JC) {
+    const int jc = min(JC, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < nk * jc; idx += NT) {
+      int k = idx / jc, jj = idx % jc;
+      Xs[k][jj] = X[(int64_t)(s.fc + k0 + k) * nrhs + j0 + jj];
+    }
+    __syncthreads();
+    if (a < na) {
+      for (int jj = g; jj < jc; jj += 4) {
+        T acc = T(0);
+        for (int k = 0; k < nk; ++k) acc += Ps[k][a] * Xs[k][jj];
+        atomic_add(Y + (int64_t)row * nrhs + j0 + jj, -acc);
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_soff_bwd(const SnInfo* __restrict__ sn,
+                                                  const Task4* __restrict__ tasks,
+                                                  const T* __restrict__ arena,
+                                                  const int32_t* __restrict__ lindx0,
+                                                  T* __restrict__ X, int nrhs) {
+  __shared__ T Ps[NB][TILE + 1];  // Ps[k][a]
+  __shared__ T Xs[TILE][JC + 1];  // gathered solution rows
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int m = s.r - s.w;
+  const int a0 = t.b * TILE, k0 = t.c * NB;
+  const int na = min(TILE, m - a0), nk = min(NB, s.w - k0);
+  for (int idx = tid; idx < NB * TILE; idx += NT) {
+    int a = idx % TILE, k = idx / TILE;
+    Ps[k][a] = (a < na && k < nk) ? P[(int64_t)(k0 + k) * ld + s.w + a0 + a] : T(0);
+  }
+  const int k = tid % NB, g = tid / NB;
+  for (int j0 = 0; j0 < nrhs; j0 += JC) {
+    const int jc = min(JC, nrhs - j0);
+    __syncthreads();
+    for (int idx = tid; idx < na * jc; idx += NT) {
+      int a = idx / jc, jj = idx % jc;
+      int row = lindx0[s.lx + s.w + a0 + a];
+      Xs[a][jj] = X[(int64_t)row * nrhs + j0 + jj];
+    }
+    __syncthreads();
+    if (k < nk) {
+      for (int jj = g; jj < jc; jj += 4) {
+        T acc = T(0);
+        for (int a = 0; a < na; ++a) acc += Ps[k][a] * Xs[a][jj];
+        atomic_add(X + (int64_t)(s.fc + k0 + k) * nrhs + j0 + jj, -acc);
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_scale_d(const SnInfo* __restrict__ sn,
+                                                 const Task4* __restrict__ tasks,
+                                                 const T* __restrict__ arena, T* __restrict__ X,
+                                                 int nrhs) {
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const T* __restrict__ P = arena + s.off;
+  for (int64_t idx = threadIdx.x; idx < (int64_t)s.w * nrhs; idx += NT) {
+    int i = (int)(idx / nrhs);
+    X[(int64_t)s.fc * nrhs + idx] /= P[(int64_t)i * s.r + i];
+  }
+}
+
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) void k_sdiag_bwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   T* __restrict__ X, int nrhs) {
+  __shared__ T Ld[NB][NB + 1];
+  __shared__ T Z[NB][JC + 1];
+  __shared__ T invd[NB];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int tid = threadIdx.x;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int64_t base = (int64_t)s.fc * nrhs;
+  const int nb = (s.w + NB - 1) / NB;
+  for (int blk = nb - 1; blk >= 0; --blk) {
+    const int c0 = blk * NB;
+    const int bs = min(NB, s.w - c0);
+    __syncthreads();
+    for (int idx = tid; idx < bs * bs; idx += NT) {
+      int i = idx % bs, j = idx / bs;
+      if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
+    }
+    __syncthreads();
+    if (tid < bs) invd[tid] = LDL ? T(1) : T(1) / Ld[tid][tid];
+    for (int j0 = 0; j0 < nrhs; j0 += JC) {
+      const int jc = min(JC, nrhs - j0);
+      // z_blk -= L[rows below the block inside the supernode, blk]^T x_below
+      const int below = s.w - c0 - bs;
+      for (int idx = tid; idx < bs * jc; idx += NT) {
+        int kq = idx / jc, jj = idx % jc;
+        T acc = T(0);
+        for (int i = 0; i < below; ++i)
+          acc += P[(int64_t)(c0 + kq) * ld + c0 + bs + i] *
+                 X[base + (int64_t)(c0 + bs + i) * nrhs + j0 + jj];
+        Z[kq][jj] = X[base + (int64_t)(c0 + kq) * nrhs + j0 + jj] - acc;
+      }
+      __syncthreads();
+      for (int k = bs - 1; k >= 0; --k) {
+        for (int idx = tid; idx < k * jc; idx += NT) {
+          int i = idx / jc, jj = idx % jc;
+          Z[i][jj] -= Ld[k][i] * (Z[k][jj] * invd[k]);
+        }
+        __syncthreads();
+      }
+      for (int idx = tid; idx < bs * jc; idx += NT) {
+        int i = idx / jc, jj = idx % jc;
+        X[base + (int64_t)(c0 + i) * nrhs + j0 + jj] = Z[i][jj] * invd[i];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// b (column-major, original numbering) -> Y (row-major, permuted numbering)
+template <typename T>
+__global__ void k_gather_rhs(const double* __restrict__ b, T* __restrict__ Y,
+                             const int32_t* __restrict__ perm0, int n, int nrhs,
+                             const double* __restrict__ esc_row) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
+    int o = perm0[i];
+    double v = b[(int64_t)j * n + o];
+    if (esc_row) v *= esc_row[o];
+    Y[idx] = (T)v;
+  }
+}
+
+// X (row-major, permuted) -> x (column-major, original numbering)
+template <typename T>
+__global__ void k_scatter_sol(const T* __restrict__ X, double* __restrict__ x,
+                              const int32_t* __restrict__ perm0, int n, int nrhs,
+                              const double* __restrict__ esc_row, int accumulate) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
+    int o = perm0[i];
+    double v = (double)X[idx];
+    if (esc_row) v *= esc_row[o];
+    double* dst = x + (int64_t)j * n + o;
+    if (accumulate)
+      *dst += v;
+    else
+      *dst = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = b - A x with A given by its lower triangle (CSC, 0-based)
+// in the original numbering, as check_solution does (driver/utils.hpp:95-142):
+// one workgroup-stride loop per column, symmetric contributions by atomics.
+// ---------------------------------------------------------------------------
+__global__ void k_residual(const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+                           const double* __restrict__ val, const double* __restrict__ x,
+                           double* __restrict__ r, int n, int nrhs) {
+  int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  for (int j = 0; j < nrhs; ++j) {
+    const double* xj = x + (int64_t)j * n;
+    double* rj = r + (int64_t)j * n;
+    double xc = xj[c];
+    double acc = 0.0;
+    for (int64_t p = colptr[c] + lane; p < colptr[c + 1]; p += 64) {
+      int row = rowind[p];
+      double v = val[p];
+      if (row != c) {
+        unsafeAtomicAdd(rj + row, -v * xc);  // A(row,c) x(c)
+        acc += v * xj[row];             // A(c,row) x(row)
+      } else {
+        acc += v * xc;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) unsafeAtomicAdd(rj + c, -acc);
+  }
+}
+
+}  // namespace dev
+}  // namespace spk

@@ -1,0 +1,208 @@
+// Host construction of the device plan (see plan.hpp).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace spk {
+
+namespace {
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+}  // namespace
+
+void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
+                Plan& p) {
+  p = Plan();
+  p.n = n;
+  const Int ns = s.nsuper();
+  p.nsuper = ns;
+  p.sn.resize(ns);
+  p.lindx0.resize(s.lindx.size());
+  for (size_t i = 0; i < s.lindx.size(); ++i) p.lindx0[i] = s.lindx[i] - 1;
+  p.perm0.resize(n);
+  for (Int i = 0; i < n; ++i) p.perm0[i] = s.ord.perm[i] - 1;
+
+  // panels, column-major, leading dimension r
+  int64_t off = 0, upd = 0;
+  for (Int I = 0; I < ns; ++I) {
+    SnInfo& si = p.sn[I];
+    si.fc = s.xsuper[I] - 1;
+    si.w = s.xsuper[I + 1] - s.xsuper[I];
+    si.lx = (int32_t)(s.xlindx[I] - 1);
+    si.r = (int32_t)(s.xlindx[I + 1] - s.xlindx[I]);
+    si.off = off;
+    si.upd = upd;
+    off += (int64_t)si.r * si.w;
+    upd += si.r - si.w;
+    p.max_w = std::max(p.max_w, (int)si.w);
+  }
+  p.arena_elems = off;
+
+  // etree heights (children have smaller indices than parents: postorder)
+  p.level.assign(ns, 0);
+  for (Int I = 0; I < ns; ++I) {
+    Int par = s.sparent[I];
+    if (par > 0) p.level[par - 1] = std::max(p.level[par - 1], p.level[I] + 1);
+  }
+  p.nlevels = 0;
+  for (Int I = 0; I < ns; ++I) p.nlevels = std::max(p.nlevels, p.level[I] + 1);
+
+  // row -> supernode (0-based) and position maps for the relative indices
+  std::vector<int32_t> row_sup(n);
+  for (Int i = 0; i < n; ++i) row_sup[i] = s.supMembership[i] - 1;
+
+  // Schur scatter maps.  For off-diagonal row b of I with target supernode J
+  // (the supernode of column lindx[w+b]) the rows a >= first row of that
+  // target group are all in struct(J); rel stores their positions.
+  p.tcol.assign(upd, 0);
+  p.tmap.assign(upd, 0);
+  p.rel.clear();
+  std::vector<int32_t> pos(n, -1);
+  for (Int I = 0; I < ns; ++I) {
+    const SnInfo& si = p.sn[I];
+    const int32_t* st = &p.lindx0[si.lx];
+    int32_t m = si.r - si.w;
+    int32_t b = 0;
+    while (b < m) {
+      int32_t row = st[si.w + b];
+      int32_t J = row_sup[row];
+      const SnInfo& sj = p.sn[J];
+      // group = consecutive off-diagonal rows falling in J's columns
+      int32_t gb = b;
+      while (b < m && row_sup[st[si.w + b]] == J) ++b;
+      int32_t ge = b;
+      // positions of struct(J)
+      const int32_t* stJ = &p.lindx0[sj.lx];
+      for (int32_t q = 0; q < sj.r; ++q) pos[stJ[q]] = q;
+      int64_t base = (int64_t)p.rel.size();
+      for (int32_t a = gb; a < m; ++a) {
+        int32_t pa = pos[st[si.w + a]];
+        if (pa < 0)
+          throw std::logic_error("symbolic structure is not closed under elimination");
+        p.rel.push_back(pa);
+      }
+      for (int32_t q = 0; q < sj.r; ++q) pos[stJ[q]] = -1;
+      for (int32_t bb = gb; bb < ge; ++bb) {
+        int32_t col = st[si.w + bb] - sj.fc;
+        p.tcol[si.upd + bb] = sj.off + (int64_t)col * sj.r;
+        p.tmap[si.upd + bb] = base - gb;
+      }
+    }
+  }
+
+  // Assembly map: caller entry (row,col) of the lower triangle goes to
+  // (max,min) of the permuted indices; later duplicates win.
+  {
+    Ptr nnz = colptr[n] - baseval;
+    p.amap.assign(nnz, -1);
+    // position of a permuted row inside struct(J): binary search
+    auto find_pos = [&](const SnInfo& sj, int32_t row) -> int32_t {
+      const int32_t* b = &p.lindx0[sj.lx];
+      const int32_t* e = b + sj.r;
+      const int32_t* it = std::lower_bound(b, e, row);
+      if (it == e || *it != row) return -1;
+      return (int32_t)(it - b);
+    };
+    for (Int c = 0; c < n; ++c) {
+      for (Ptr q = colptr[c] - baseval; q < colptr[c + 1] - baseval; ++q) {
+        int32_t r0 = rowind[q] - baseval;
+        int32_t pr = s.ord.invp[r0] - 1, pc = s.ord.invp[c] - 1;
+        int32_t trow = std::max(pr, pc), tcol = std::min(pr, pc);
+        const SnInfo& sj = p.sn[row_sup[tcol]];
+        int32_t pp = find_pos(sj, trow);
+        if (pp < 0) throw std::logic_error("matrix entry outside the symbolic structure");
+        p.amap[q] = sj.off + (int64_t)(tcol - sj.fc) * sj.r + pp;
+      }
+    }
+    // resolve duplicates: keep the last occurrence
+    std::vector<std::pair<int64_t, int64_t>> keyed(nnz);
+    for (Ptr q = 0; q < nnz; ++q) keyed[q] = {p.amap[q], q};
+    std::sort(keyed.begin(), keyed.end());
+    for (Ptr q = 0; q + 1 < nnz; ++q)
+      if (keyed[q].first == keyed[q + 1].first) p.amap[keyed[q].second] = -1;
+  }
+
+  // ---- factorization task lists, level by level ----
+  std::vector<std::vector<Int>> by_level(p.nlevels);
+  for (Int I = 0; I < ns; ++I) by_level[p.level[I]].push_back(I);
+  for (int L = 0; L < p.nlevels; ++L) {
+    const auto& sns = by_level[L];
+    int smax = 0;
+    for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kNB));
+    for (int st = 0; st < smax; ++st) {
+      LaunchRange pr{0, (int64_t)p.potrf_tasks.size(), 0};
+      LaunchRange tr{7, (int64_t)p.trsm_tasks.size(), 0};
+      LaunchRange ur{1, (int64_t)p.inpanel_tasks.size(), 0};
+      for (Int I : sns) {
+        const SnInfo& si = p.sn[I];
+        int nb = (int)ceil_div(si.w, kNB);
+        if (st >= nb) continue;
+        int c0 = st * kNB, bs = std::min(kNB, si.w - c0);
+        int64_t rows_below = si.r - c0 - bs;
+        p.potrf_tasks.push_back({I, st, 0, 0});
+        for (int64_t rt = 0; rt < ceil_div(rows_below, kTile); ++rt)
+          p.trsm_tasks.push_back({I, st, (int32_t)rt, 0});
+        p.diag_flops += (double)bs * bs * bs / 3.0 + (double)bs * bs * rows_below;
+        if (st < nb - 1) {
+          int rb = c0 + bs;  // trailing rows/cols start
+          int64_t nrow = si.r - rb, ncol = si.w - rb;
+          int64_t ntr = ceil_div(nrow, kTile), ntc = ceil_div(ncol, kTile);
+          for (int64_t ti = 0; ti < ntr; ++ti)
+            for (int64_t tj = 0; tj < ntc && tj <= ti; ++tj)
+              p.inpanel_tasks.push_back({I, st, (int32_t)ti, (int32_t)tj});
+          // flops of the lower-trapezoid update with k = bs
+          p.inpanel_flops += (double)bs * ((double)ncol * (ncol + 1) + 2.0 * (nrow - ncol) * ncol);
+        }
+      }
+      pr.count = (int64_t)p.potrf_tasks.size() - pr.first;
+      tr.count = (int64_t)p.trsm_tasks.size() - tr.first;
+      ur.count = (int64_t)p.inpanel_tasks.size() - ur.first;
+      if (pr.count) p.factor_seq.push_back(pr);
+      if (tr.count) p.factor_seq.push_back(tr);
+      if (ur.count) p.factor_seq.push_back(ur);
+    }
+    LaunchRange sr{2, (int64_t)p.schur_tasks.size(), 0};
+    for (Int I : sns) {
+      const SnInfo& si = p.sn[I];
+      int64_t m = si.r - si.w;
+      if (m <= 0) continue;
+      int64_t nt = ceil_div(m, kTile);
+      for (int64_t ti = 0; ti < nt; ++ti)
+        for (int64_t tj = 0; tj <= ti; ++tj) p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, 0});
+      p.schur_flops += (double)si.w * (double)m * (double)(m + 1);
+      p.schur_entries += m * (m + 1) / 2;
+    }
+    sr.count = (int64_t)p.schur_tasks.size() - sr.first;
+    if (sr.count) p.factor_seq.push_back(sr);
+  }
+
+  // ---- solve task lists ----
+  // Per level: one diagonal task per supernode (bwd_tasks holds {sn}) and
+  // one off-diagonal task per 64-row tile of the panel below the diagonal.
+  std::vector<int64_t> dfirst(p.nlevels), ofirst(p.nlevels), ocount(p.nlevels);
+  for (int L = 0; L < p.nlevels; ++L) {
+    const auto& sns = by_level[L];
+    dfirst[L] = (int64_t)p.sdiag_tasks.size();
+    for (Int I : sns) p.sdiag_tasks.push_back({I, 0, 0, 0});
+    ofirst[L] = (int64_t)p.soff_tasks.size();
+    for (Int I : sns) {
+      int64_t m = p.sn[I].r - p.sn[I].w;
+      int64_t nkt = ceil_div(p.sn[I].w, kNB);
+      for (int64_t rt = 0; rt < ceil_div(m, kTile); ++rt)
+        for (int64_t kt = 0; kt < nkt; ++kt) p.soff_tasks.push_back({I, (int32_t)rt, (int32_t)kt, 0});
+    }
+    ocount[L] = (int64_t)p.soff_tasks.size() - ofirst[L];
+  }
+  // forward: levels bottom-up, diagonal solves then off-diagonal updates.
+  for (int L = 0; L < p.nlevels; ++L) {
+    p.fwd_seq.push_back({3, dfirst[L], (int64_t)by_level[L].size()});
+    if (ocount[L]) p.fwd_seq.push_back({4, ofirst[L], ocount[L]});
+  }
+  // backward: levels top-down, off-diagonal gathers then diagonal solves.
+  for (int L = p.nlevels - 1; L >= 0; --L) {
+    if (ocount[L]) p.bwd_seq.push_back({5, ofirst[L], ocount[L]});
+    p.bwd_seq.push_back({6, dfirst[L], (int64_t)by_level[L].size()});
+  }
+}
+
+}  // namespace spk

@@ -1,0 +1,81 @@
+// Device execution plan for the supernodal factorization and solve.
+//
+// The plan is built once per symbolic structure on the host and uploaded to
+// HBM.  It fixes:
+//  * the factor layout: one contiguous arena holding every supernode's panel
+//    (all rows of its structure x its columns), column-major with leading
+//    dimension = number of structure rows, so a column of a panel is a
+//    contiguous run of HBM;
+//  * relative-index maps, so the Schur-update kernel scatters into ancestor
+//    panels without searching block lists;
+//  * the assembly map from the caller's CSC entries to arena offsets;
+//  * etree-level task lists for every batched kernel launch.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../analysis/analysis.hpp"
+
+namespace spk {
+
+constexpr int kNB = 64;    // diagonal block width of the blocked panel factorization
+constexpr int kTile = 64;  // update tile edge (rows x cols)
+
+struct SnInfo {            // mirrored on the device (16-byte aligned)
+  int64_t off;             // panel offset in the arena (elements)
+  int64_t upd;             // offset into tcol/tmap for the panel's off-diagonal rows
+  int32_t r;               // structure rows (= panel leading dimension)
+  int32_t w;               // width (columns)
+  int32_t fc;              // first column, 0-based, permuted numbering
+  int32_t lx;              // offset of the structure in lindx0
+};
+
+struct Task4 {
+  int32_t a, b, c, d;
+};
+
+struct LaunchRange {
+  int kind;          // 0 diagonal factor, 7 panel TRSM, 1 in-panel update, 2 Schur update,
+                     // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
+  int64_t first;     // first task in that kind's task array
+  int64_t count;     // number of tasks (= grid size)
+};
+
+struct Plan {
+  int32_t n = 0, nsuper = 0;
+  int64_t arena_elems = 0;
+  std::vector<SnInfo> sn;
+  std::vector<int32_t> lindx0;     // 0-based structure rows (permuted numbering)
+  std::vector<int32_t> level;      // etree height of every supernode
+  int32_t nlevels = 0;
+  // Schur scatter maps (per off-diagonal row b of supernode I):
+  // target address of element (a,b) = tcol[upd+b] + rel[tmap[upd+b] + a].
+  std::vector<int64_t> tcol;
+  std::vector<int64_t> tmap;
+  std::vector<int32_t> rel;
+  // assembly: arena offset of every caller CSC entry (-1 = superseded duplicate)
+  std::vector<int64_t> amap;
+  std::vector<int32_t> perm0;      // perm0[new] = old, 0-based
+  // task arrays
+  std::vector<Task4> potrf_tasks;  // {sn, step, 0, 0}
+  std::vector<Task4> trsm_tasks;   // {sn, step, row_tile, 0}
+  std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
+  std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}
+  std::vector<LaunchRange> factor_seq;
+  // solve
+  std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
+  std::vector<Task4> soff_tasks;   // {sn, row_tile, k_tile, 0}: off-diagonal panel tiles
+  std::vector<LaunchRange> fwd_seq, bwd_seq;
+  // algorithmic counts
+  double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
+  int64_t schur_entries = 0;       // lower-triangle entries scattered
+  int max_w = 0;
+};
+
+// Build the plan from the symbolic analysis and the caller's CSC pattern
+// (lower triangle, any base).
+void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
+                Plan& p);
+
+}  // namespace spk
