@@ -82,6 +82,9 @@ typedef struct spk_stats {
  * 4 forward off-diagonal, 5 backward off-diagonal, 6 backward diagonal. */
 int spk_get_kernel_stats(spk_handle h, int kind, int64_t* launches, double* ms);
 int spk_reset_kernel_stats(spk_handle h);
+/* Switch per-launch event timing on/off after spk_init (no reference
+ * counterpart; the reference's timing is the SPROFILE build, timer.hpp:56-85). */
+int spk_set_profile(spk_handle h, int on);
 
 /* Library set-up, as the reference's C entry points (include/sympack.hpp:31-36,
  * src/global.cpp:114-185).  One rank: symPACK_Rank always returns 0. */

@@ -324,6 +324,12 @@ int spk_get_kernel_stats(spk_handle h, int kind, int64_t* launches, double* ms) 
   return 0;
 }
 
+int spk_set_profile(spk_handle h, int on) {
+  if (!h) return fail("null handle");
+  h->engine.set_profile(on != 0);
+  return 0;
+}
+
 int spk_reset_kernel_stats(spk_handle h) {
   if (!h) return fail("null handle");
   h->engine.reset_stats();

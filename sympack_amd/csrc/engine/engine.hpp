@@ -46,6 +46,7 @@ class Engine {
 
   void init(const EngineOptions& o);
   void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
+  void set_profile(bool on) { opt_.profile = on ? 1 : 0; }
   void symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval);
   void distribute(const double* nzval, bool on_device);
   int factorize();

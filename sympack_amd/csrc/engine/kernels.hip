@@ -36,12 +36,16 @@ constexpr int NT = 256;
 
 template <typename T>
 struct Mfma;
+// Accumulator layout (lane l, slot v) -> row i of the 16x16 block; the column
+// is l%16 for both.  Measured on gfx950: the f64 instruction interleaves rows
+// (i = 4v + l/16), the f32 one blocks them (i = 4(l/16) + v).
 template <>
 struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 template <>
 struct Mfma<float> {
@@ -49,6 +53,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 template <typename T>
@@ -187,9 +192,9 @@ __global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
 // MFMA tile update.  The 64x64 output tile C[a][b] = sum_k Pa[a][k] Pb[b][k]
 // (times D[k] for LDL) is computed by 4 waves, each owning a 32x32 quarter as
 // 2x2 blocks of 16x16x4 MFMA.  MFMA operand A is the b side and operand B the
-// a side, so an accumulator element (lane l, slot v) holds
-// b = 4*(l/16)+v, a = l%16 within its block: 16 consecutive lanes own 16
-// consecutive target rows, which keeps the scatter coalesced.
+// a side, so an accumulator element (lane l, slot v) holds b = Mfma<T>::row(l,v)
+// and a = l%16 within its block: 16 consecutive lanes own 16 consecutive
+// target rows, which keeps the scatter coalesced.
 // MODE 0: in-panel trailing update of step t.b (direct read-modify-write).
 // MODE 1: Schur update of ancestor panels (atomic scatter).
 // ---------------------------------------------------------------------------
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
       for (int aj = 0; aj < 2; ++aj)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int bg = brow0 + wb * 32 + bi * 16 + 4 * (lane >> 4) + v;
+          const int bg = brow0 + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
           const int ag = arow0 + wa * 32 + aj * 16 + (lane & 15);
           if (ag < s.r && bg < s.w && ag >= bg) P[(int64_t)bg * ld + ag] -= acc[bi][aj][v];
         }
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int boff = tj * TILE + wb * 32 + bi * 16 + 4 * (lane >> 4) + v;
+        const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
         if (boff >= m) continue;
         const int64_t tc = tcol[s.upd + boff];
         const int64_t tm = tmap[s.upd + boff];

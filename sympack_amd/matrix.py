@@ -248,6 +248,9 @@ class symPACKMatrix2D:
     def reset_kernel_stats(self):
         self._lib.spk_reset_kernel_stats(self._h)
 
+    def set_profile(self, on):
+        _lib.check(self._lib.spk_set_profile(self._h, 1 if on else 0), "set_profile")
+
     def TotalSupernodeCnt(self):
         return self.stats()["nsuper"]
 

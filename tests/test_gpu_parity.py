@@ -1,0 +1,128 @@
+"""GPU parity: HIP factor and solve against the oracle (oracle/supernodal.c)
+on the same matrices, orderings and symbolic structure.
+
+Tolerances: factor panels agree to 1e-10 relative (Frobenius, FP64 with a
+different summation order); solutions within 1e-10 relative of the oracle's;
+backward error within 10x of the oracle's (the drop-in bar).
+"""
+import numpy as np
+import pytest
+
+import sympack_amd as sp
+from sympack_amd import matrices as M
+from oracle import numeric as onum
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(gen, ordering, ldl=False, nrhs=1, relax_max=None, seed=0):
+    n, cp, ri, v = gen()
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    if relax_max is not None:
+        rp.SetMaxSize(relax_max)
+    opts = sp.symPACKOptions(ordering=ordering, decomposition="LDL" if ldl else "LL", relax=rp)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(opts)
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    print("case n=%d ordering=%s ldl=%s nrhs=%d nsuper=%d" % (n, ordering, ldl, nrhs,
+                                                             Mx.TotalSupernodeCnt()), flush=True)
+    Mx.Factorize()
+    rng = np.random.default_rng(seed)
+    xt = rng.uniform(-1, 1, (n, nrhs))
+    B = A.matvec(xt)
+    Mx.Solve(B, nrhs)
+    X = Mx.GetSolution(nrhs).reshape(n, nrhs)
+    S = onum.Structure.of(Mx)
+    info, F = onum.factor(S, A, ldl)
+    assert info == 0
+    Xo = onum.solve(S, F, B, ldl)
+    return A, Mx, S, F, X, Xo, B, xt
+
+
+CASES = [
+    ("lap2d_64_mmd", lambda: M.laplacian_2d_5pt(64), "MMD", None),
+    ("lap2d_40_ndgrid", lambda: M.laplacian_2d_5pt(40), "NDGRID", None),
+    ("diff3d_12_ndbox_b150", lambda: M.diffusion_3d_7pt(12, seed=3), "NDBOX", 150),
+    ("diff3d_16_ndbox_b150", lambda: M.diffusion_3d_7pt(16, seed=4), "NDBOX", 150),
+    ("diff3d_10_ndbox_unlimited", lambda: M.diffusion_3d_7pt(10, seed=5), "NDBOX", None),
+    ("st27_10_mmd", lambda: M.stencil_3d_27pt(10, seed=6), "MMD", None),
+    ("rand_spd_600_rcm", lambda: M.random_spd(600, 0.01, 7), "RCM", None),
+]
+
+
+@pytest.mark.parametrize("name,gen,ordering,bmax", CASES, ids=[c[0] for c in CASES])
+def test_factor_panels_and_solve_ll(name, gen, ordering, bmax):
+    A, Mx, S, F, X, Xo, B, xt = _run(gen, ordering, relax_max=bmax, nrhs=2)
+    op = onum.panels(S, F)
+    num = den = 0.0
+    for I in range(1, S.nsuper + 1):
+        g = Mx.GetFactorPanel(I)
+        o = op[I - 1]
+        w = g.shape[1]
+        # the strict upper part of the diagonal block is not part of L
+        mask = np.ones_like(o, bool)
+        mask[:w, :w] = np.tril(np.ones((w, w), bool))
+        num += np.sum((g[mask] - o[mask]) ** 2)
+        den += np.sum(o[mask] ** 2)
+    assert np.sqrt(num / den) < 1e-10
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
+    be_g = onum.backward_error(A, X, B)
+    be_o = onum.backward_error(A, Xo, B)
+    assert be_g <= 10 * max(be_o, 1e-16)
+
+
+@pytest.mark.parametrize("name,gen,ordering,bmax", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_ldl_matches_oracle(name, gen, ordering, bmax):
+    A, Mx, S, F, X, Xo, B, xt = _run(gen, ordering, ldl=True, relax_max=bmax, nrhs=3)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
+    assert onum.backward_error(A, X, B) <= 10 * max(onum.backward_error(A, Xo, B), 1e-16)
+
+
+def test_indefinite_ldl_shifted_laplacian():
+    A, Mx, S, F, X, Xo, B, xt = _run(lambda: M.shifted_laplacian_3d_7pt(12, 0.3), "NDBOX",
+                                     ldl=True, relax_max=150, nrhs=4)
+    # static pivots on an indefinite matrix: same bar as the drop-in, relative
+    # to the oracle's own backward error
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-9
+    assert onum.backward_error(A, X, B) <= 10 * max(onum.backward_error(A, Xo, B), 1e-16)
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 16, 64, 70])
+def test_many_rhs(nrhs):
+    A, Mx, S, F, X, Xo, B, xt = _run(lambda: M.diffusion_3d_7pt(10, seed=9), "NDBOX",
+                                     relax_max=150, nrhs=nrhs)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
+
+
+def test_not_spd_raises():
+    n, cp, ri, v = M.laplacian_2d_5pt(16)
+    v = v.copy()
+    v[cp[100]] = -5.0  # make one diagonal entry negative
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="MMD"))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    with pytest.raises(sp.NotHPDError):
+        Mx.Factorize()
+
+
+def test_fp32_ir_reaches_fp64_accuracy():
+    n, cp, ri, v = M.stencil_3d_27pt(12, seed=11)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision="FP32_IR", ir_tol=1e-12))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((n, 4))
+    Mx.Solve(B, 4)
+    X = Mx.GetSolution(4)
+    st = Mx.stats()
+    assert st["backward_error"] <= 1e-12
+    assert onum.backward_error(A, X, B) <= 1e-12
