@@ -85,6 +85,9 @@ int spk_reset_kernel_stats(spk_handle h);
 /* Switch per-launch event timing on/off after spk_init (no reference
  * counterpart; the reference's timing is the SPROFILE build, timer.hpp:56-85). */
 int spk_set_profile(spk_handle h, int on);
+/* Per-launch records of the profiled calls since the last reset: kind, grid
+ * size (workgroups) and event time; returns the number of records (<= max). */
+int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max);
 
 /* Library set-up, as the reference's C entry points (include/sympack.hpp:31-36,
  * src/global.cpp:114-185).  One rank: symPACK_Rank always returns 0. */

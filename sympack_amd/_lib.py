@@ -88,6 +88,7 @@ SIGNATURES = [
     ("spk_get_kernel_stats", _I, [_P, _I, _PI64, _PD]),
     ("spk_reset_kernel_stats", _I, [_P]),
     ("spk_set_profile", _I, [_P, _I]),
+    ("spk_get_launch_trace", _I, [_P, _PI32, _PI64, _PD, _I]),
     ("spk_get_supernodal_partition", _I, [_P, _PI32]),
     ("spk_get_sup_membership", _I, [_P, _PI32]),
     ("spk_get_ordering", _I, [_P, _PI32, _PI32]),

@@ -330,6 +330,18 @@ int spk_set_profile(spk_handle h, int on) {
   return 0;
 }
 
+int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max) {
+  if (!h) return fail("null handle");
+  const auto& t = h->engine.trace;
+  int cnt = (int)std::min<size_t>(t.size(), (size_t)std::max(0, max));
+  for (int i = 0; i < cnt; ++i) {
+    if (kinds) kinds[i] = t[i].kind;
+    if (grids) grids[i] = t[i].grid;
+    if (ms) ms[i] = t[i].ms;
+  }
+  return cnt;
+}
+
 int spk_reset_kernel_stats(spk_handle h) {
   if (!h) return fail("null handle");
   h->engine.reset_stats();

@@ -95,6 +95,7 @@ void Engine::init(const EngineOptions& o) {
 
 void Engine::reset_stats() {
   for (auto& k : kind_stats) k = KindStats();
+  trace.clear();
 }
 
 void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) {
@@ -213,9 +214,10 @@ void Engine::assemble(T* arena) {
   HIP_OK(hipGetLastError());
 }
 
-void Engine::launch_begin(int kind) {
+void Engine::launch_begin(int kind, int64_t grid) {
   kind_stats[kind].launches++;
   if (!opt_.profile) return;
+  ev_grid_.push_back(grid);
   while (ev_pool_.size() < ev_next_ + 2) {
     hipEvent_t e;
     HIP_OK(hipEventCreate(&e));
@@ -235,12 +237,15 @@ void Engine::launch_end(int kind) {
 
 void Engine::collect_events() {
   if (!opt_.profile) return;
-  for (auto& o : ev_open_) {
+  for (size_t i = 0; i < ev_open_.size(); ++i) {
+    auto& o = ev_open_[i];
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, o.second.first, o.second.second));
     kind_stats[o.first].ms += ms;
+    if (trace.size() < 100000) trace.push_back({o.first, i < ev_grid_.size() ? ev_grid_[i] : 0, ms});
   }
   ev_open_.clear();
+  ev_grid_.clear();
   ev_next_ = 0;
 }
 
@@ -250,14 +255,14 @@ void Engine::factor_sequence(T* arena) {
   const bool ldl = opt_.ldl != 0;
   for (const LaunchRange& r : plan_.factor_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind);
+    launch_begin(r.kind, r.count);
     switch (r.kind) {
       case 0: {
         const Task4* t = (const Task4*)d_tasks_[TA_POTRF] + r.first;
         if (ldl)
-          dev::k_potrf<T, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_info_);
+          dev::k_potrf<T, true><<<g, 64, 0, stream_>>>(sn, t, arena, d_info_);
         else
-          dev::k_potrf<T, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_info_);
+          dev::k_potrf<T, false><<<g, 64, 0, stream_>>>(sn, t, arena, d_info_);
         break;
       }
       case 7: {
@@ -340,7 +345,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   const Task4* so = (const Task4*)d_tasks_[TA_SOFF];
   for (const LaunchRange& r : plan_.fwd_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind);
+    launch_begin(r.kind, r.count);
     if (r.kind == 3) {
       if (ldl)
         dev::k_sdiag_fwd<T, true><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
@@ -358,7 +363,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   }
   for (const LaunchRange& r : plan_.bwd_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind);
+    launch_begin(r.kind, r.count);
     if (r.kind == 5) {
       dev::k_soff_bwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, nrhs);
     } else {

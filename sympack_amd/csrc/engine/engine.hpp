@@ -60,6 +60,12 @@ class Engine {
   // copy the panel of supernode I (0-based) to host, as double
   void get_panel(Int I, double* out) const;
   std::vector<KindStats> kind_stats;  // indexed by LaunchRange::kind
+  struct LaunchRecord {
+    int kind;
+    int64_t grid;
+    double ms;
+  };
+  std::vector<LaunchRecord> trace;    // per launch of the last profiled calls
   void reset_stats();
   double last_factor_ms = 0.0, last_solve_ms = 0.0;
   SolveReport last_report;
@@ -74,7 +80,7 @@ class Engine {
   template <typename T>
   void assemble(T* arena);
   void residual(const double* b, const double* x, double* r, int nrhs);
-  void launch_begin(int kind);
+  void launch_begin(int kind, int64_t grid = 0);
   void launch_end(int kind);
 
   EngineOptions opt_;
@@ -118,6 +124,7 @@ class Engine {
   // profiling events
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
+  std::vector<int64_t> ev_grid_;
   size_t ev_next_ = 0;
   void collect_events();
 };

@@ -61,6 +61,21 @@ __device__ __forceinline__ void atomic_add(T* p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
+// Marks a register as used at this point.  Placed after a batch of
+// unconditional loads it keeps the compiler from sinking each load into its
+// own predicated region (which would wait for every load separately): all
+// loads are issued back to back and waited for once.
+template <typename T>
+__device__ __forceinline__ void keep(T& v) {
+  if constexpr (sizeof(T) == 8)
+    asm volatile("" : "+v"(v));
+  else
+    asm volatile("" : "+v"(v));
+}
+// Compiler-only memory barrier: stops loads of later steps being hoisted
+// above this point (which would raise register pressure and spill).
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
 // ---------------------------------------------------------------------------
 // Zero fill / assembly
 // ---------------------------------------------------------------------------
@@ -88,71 +103,81 @@ __global__ void k_assemble(T* __restrict__ arena, const int64_t* __restrict__ am
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block factorization (bs <= 64) in LDS.  Task {sn, step, -, -}.
+// Diagonal block factorization (bs <= 64), one wave per block.  Lane i keeps
+// row i of the block in registers; at step j the unscaled pivot column is
+// broadcast through LDS and every lane applies the rank-1 update
+// a_ik -= a_ij a_kj / a_jj to its own row (the same formula serves LL^T and
+// LDL^T); the column is scaled at the end of the step.  Task {sn, step, -, -};
 // info receives the smallest failing pivot column (1-based, permuted order).
 // ---------------------------------------------------------------------------
 template <typename T, bool LDL>
-__global__ __launch_bounds__(NT) void k_potrf(const SnInfo* __restrict__ sn,
+__global__ __launch_bounds__(64) void k_potrf(const SnInfo* __restrict__ sn,
                                                const Task4* __restrict__ tasks,
                                                T* __restrict__ arena, int* __restrict__ info) {
-  __shared__ T Ld[NB][NB + 1];
+  __shared__ __attribute__((aligned(16))) T col[NB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int c0 = t.b * NB;
   const int bs = min(NB, s.w - c0);
   T* __restrict__ P = arena + s.off;
   const int64_t ld = s.r;
-  for (int idx = tid; idx < bs * bs; idx += NT) {
-    int i = idx % bs, j = idx / bs;
-    if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
-  }
-  __syncthreads();
-  for (int j = 0; j < bs; ++j) {
-    // pivot column: only wave 0 (bs - j <= 64 lanes); every lane reads the
-    // pivot in the same LDS instruction before lane 0 overwrites it.
-    if (tid < bs - j) {
-      T ajj = Ld[j][j];
-      T aij = Ld[j + tid][j];
-      bool bad = LDL ? (ajj == T(0) || ajj != ajj) : !(ajj > T(0));
-      if (bad && tid == 0) atomicMin(info, s.fc + c0 + j + 1);
-      if (!LDL) {
-        T d = bad ? T(1) : sqrt(ajj);
-        Ld[j + tid][j] = (tid == 0) ? d : aij / d;
-      } else {
-        T d = bad ? T(1) : ajj;
-        if (tid > 0) Ld[j + tid][j] = aij / d;
+  // clamped, unconditional loads keep all 64 loads in flight; entries
+  // outside the block's lower triangle are zero, so the update loops below
+  // need no per-element guards.
+  T a[NB];
+  const int lr = min(lane, bs - 1);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) a[k] = P[(int64_t)(c0 + min(k, bs - 1)) * ld + c0 + lr];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) keep(a[k]);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) a[k] = (lane < bs && k <= lane) ? a[k] : T(0);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < bs) {
+      col[lane] = a[j];
+      __syncthreads();
+      T ajj = col[j];
+      const bool bad = LDL ? (ajj == T(0) || ajj != ajj) : !(ajj > T(0));
+      if (bad) {
+        if (lane == 0) atomicMin(info, s.fc + c0 + j + 1);
+        ajj = T(1);
       }
+      const T inv = T(1) / ajj;
+      const T t2 = a[j] * inv;
+#pragma unroll
+      for (int k = j + 1; k < NB; ++k) a[k] -= t2 * col[k];
+      if (LDL)
+        a[j] = (lane == j) ? ajj : t2;
+      else
+        a[j] = (lane == j) ? sqrt(ajj) : a[j] * (T(1) / sqrt(ajj));
+      __syncthreads();
     }
-    __syncthreads();
-    const int m = bs - j - 1;
-    const T dj = LDL ? Ld[j][j] : T(1);
-    for (int idx = tid; idx < m * m; idx += NT) {
-      int ii = idx % m, kk = idx / m;
-      if (ii >= kk) {
-        int i = j + 1 + ii, k = j + 1 + kk;
-        Ld[i][k] -= Ld[i][j] * Ld[k][j] * dj;
-      }
-    }
-    __syncthreads();
   }
-  for (int idx = tid; idx < bs * bs; idx += NT) {
-    int i = idx % bs, j = idx / bs;
-    if (i >= j) P[(int64_t)(c0 + j) * ld + c0 + i] = Ld[i][j];
+  if (lane < bs) {
+    T* prow = P + (int64_t)c0 * ld + c0 + lane;
+    asm volatile("" : "+v"(prow));
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+      if (k <= lane) prow[(int64_t)k * ld] = a[k];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Panel TRSM of one 64-row tile below a factored diagonal block:
-// B <- B L^{-T} (LL) or B <- B L^{-T} D^{-1} (LDL, unit L).
+// Panel TRSM of a 256-row tile below a factored diagonal block:
+// B <- B L^{-T} (LL) or B <- B L^{-T} D^{-1} (LDL, unit L).  The block is
+// staged in LDS by columns; each thread then solves its own row from
+// registers with broadcast LDS reads, without further barriers.
 // Task {sn, step, row_tile, -}.
 // ---------------------------------------------------------------------------
+constexpr int TRSM_ROWS = 256;
+
 template <typename T, bool LDL>
 __global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
                                               const Task4* __restrict__ tasks,
                                               T* __restrict__ arena) {
-  __shared__ T Ld[NB][NB + 1];
-  __shared__ T X[TILE][NB + 1];
+  __shared__ __attribute__((aligned(16))) T Lc[NB][NB];  // Lc[j][k] = L(k, j)
   __shared__ T invd[NB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
@@ -161,31 +186,46 @@ __global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
   const int bs = min(NB, s.w - c0);
   T* __restrict__ P = arena + s.off;
   const int64_t ld = s.r;
-  const int R0 = c0 + bs + t.c * TILE;
-  const int nr = min(TILE, s.r - R0);
-  if (nr <= 0) return;  // uniform across the workgroup
-  for (int idx = tid; idx < bs * bs; idx += NT) {
-    int i = idx % bs, j = idx / bs;
-    if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
-  }
-  for (int idx = tid; idx < TILE * bs; idx += NT) {
-    int rr = idx % TILE, j = idx / TILE;
-    X[rr][j] = (rr < nr) ? P[(int64_t)(c0 + j) * ld + R0 + rr] : T(0);
+  const int R0 = c0 + bs + t.c * TRSM_ROWS;
+  for (int idx = tid; idx < NB * NB; idx += NT) {
+    const int k = idx % NB, j = idx / NB;  // column j, row k: coalesced down the column
+    const T v = P[(int64_t)(c0 + min(j, bs - 1)) * ld + c0 + min(k, bs - 1)];
+    Lc[j][k] = (k < bs && j < bs && k > j) ? v : T(0);
+    if (k == j) invd[j] = (j < bs) ? T(1) / v : T(0);
   }
   __syncthreads();
-  if (tid < bs) invd[tid] = T(1) / Ld[tid][tid];
-  __syncthreads();
-  // row-wise substitution, four threads per row, one barrier per column.
-  const int rr = tid % TILE, q = tid / TILE;
-  for (int j = 0; j < bs; ++j) {
-    const T xj = LDL ? X[rr][j] : X[rr][j] * invd[j];
-    for (int k = j + 1 + q; k < bs; k += 4) X[rr][k] -= xj * Ld[k][j];
-    __syncthreads();
+  // rows past the panel work on a clamped (valid) row and do not store; all
+  // threads stay for the barriers below
+  const bool active = R0 + tid < s.r;
+  const int row = min(R0 + tid, s.r - 1);
+  // columns >= bs read column bs-1 (a valid address) and are zeroed; Lc is
+  // zero on and above its diagonal and invd is zero beyond bs, so the
+  // substitution needs no per-element guards.
+  T x[NB];
+  {
+    const T* prow = P + (int64_t)c0 * ld + row;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) x[j] = prow[(int64_t)min(j, bs - 1) * ld];
   }
-  for (int idx = tid; idx < TILE * bs; idx += NT) {
-    int r2 = idx % TILE, j = idx / TILE;
-    if (r2 < nr) P[(int64_t)(c0 + j) * ld + R0 + r2] = X[r2][j] * invd[j];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) keep(x[j]);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) x[j] = (j < bs) ? x[j] : T(0);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const T xj = LDL ? x[j] : x[j] * invd[j];
+#pragma unroll
+    for (int k = j + 1; k < NB; ++k) x[k] -= xj * Lc[j][k];
   }
+  if (!active) return;
+  // a fresh row pointer: the load addresses must not stay live until here
+  T* prow = P + (int64_t)c0 * ld + row;
+  asm volatile("" : "+v"(prow));
+  // unconditional stores in descending column order: columns >= bs write
+  // zeros to column bs-1, which its own (correct) value then overwrites; a
+  // guard per column would split the code into 64 blocks and spill
+#pragma unroll
+  for (int j = NB - 1; j >= 0; --j) prow[(int64_t)min(j, bs - 1) * ld] = x[j] * invd[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -206,7 +246,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
                                                 const int64_t* __restrict__ tmap,
                                                 const int32_t* __restrict__ rel) {
   typedef typename Mfma<T>::v4 v4;
-  constexpr int KC = 16;
+  constexpr int KC = 32;
   constexpr int LDS_LD = TILE + 16;
   __shared__ T As[KC][LDS_LD];
   __shared__ T Bs[KC][LDS_LD];
@@ -244,28 +284,38 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
+  // loader mapping: a wave reads 64 consecutive rows of one k column
   const int lrow = tid & 63, lk = tid >> 6;
   const bool arow_ok = arow0 + lrow < alim;
   const bool brow_ok = brow0 + lrow < blim;
+  const int arow = min(arow0 + lrow, alim - 1), brow = min(brow0 + lrow, blim - 1);
+  constexpr int NQ = KC / 4;
+  T ra[NQ], rb[NQ];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int kg = kc + lk + 4 * q;
+      const T* col = P + (int64_t)(kbeg + min(kg, K - 1)) * ld;
+      // clamped addresses hold finite values: masking by multiplication
+      // keeps the loads unconditional (a select would be turned into a
+      // predicated load with its own wait)
+      ra[q] = col[arow] * T((kg < K) & arow_ok);
+      rb[q] = col[brow] * T((kg < K) & brow_ok);
+    }
+  };
+  fetch(0);
   for (int kc = 0; kc < K; kc += KC) {
 #pragma unroll
-    for (int q = 0; q < KC / 4; ++q) {
-      const int k = lk + 4 * q;
-      const int kg = kc + k;
-      T av = T(0), bv = T(0);
-      if (kg < K) {
-        const T* col = P + (int64_t)(kbeg + kg) * ld;
-        if (arow_ok) av = col[arow0 + lrow];
-        if (brow_ok) bv = col[brow0 + lrow];
-      }
-      As[k][lrow] = av;
-      Bs[k][lrow] = bv;
+    for (int q = 0; q < NQ; ++q) {
+      As[lk + 4 * q][lrow] = ra[q];
+      Bs[lk + 4 * q][lrow] = rb[q];
     }
     if (LDL && tid < KC) {
       const int kg = kc + tid;
       Dk[tid] = (kg < K) ? P[(int64_t)(kbeg + kg) * ld + kbeg + kg] : T(0);
     }
     __syncthreads();
+    if (kc + KC < K) fetch(kc + KC);  // next chunk in flight during the MFMAs
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
       const int k = kk * 4 + (lane >> 4);
@@ -285,7 +335,10 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     __syncthreads();
   }
 
+  // Epilogue: every load first (clamped, unconditional, one wait), then the
+  // predicated stores / atomics.
   if (MODE == 0) {
+    T cur[2][2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -294,22 +347,75 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         for (int v = 0; v < 4; ++v) {
           const int bg = brow0 + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
           const int ag = arow0 + wa * 32 + aj * 16 + (lane & 15);
-          if (ag < s.r && bg < s.w && ag >= bg) P[(int64_t)bg * ld + ag] -= acc[bi][aj][v];
+          cur[bi][aj][v] = P[(int64_t)min(bg, s.w - 1) * ld + min(ag, s.r - 1)];
+        }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int aj = 0; aj < 2; ++aj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) keep(cur[bi][aj][v]);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int aj = 0; aj < 2; ++aj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int bg = brow0 + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+          const int ag = arow0 + wa * 32 + aj * 16 + (lane & 15);
+          if (ag < s.r && bg < s.w && ag >= bg)
+            P[(int64_t)bg * ld + ag] = cur[bi][aj][v] - acc[bi][aj][v];
         }
   } else {
     const int m = s.r - s.w;
+    int64_t tc[2][4], tm[2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
-        if (boff >= m) continue;
-        const int64_t tc = tcol[s.upd + boff];
-        const int64_t tm = tmap[s.upd + boff];
+        const int bc = min(boff, m - 1);
+        tc[bi][v] = tcol[s.upd + bc];
+        tm[bi][v] = tmap[s.upd + bc];
+      }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        keep(tc[bi][v]);
+        keep(tm[bi][v]);
+      }
+    int32_t rl[2][4][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+        const int bc = min(boff, m - 1);
 #pragma unroll
         for (int aj = 0; aj < 2; ++aj) {
           const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
-          if (aoff < m && aoff >= boff) atomic_add(arena + tc + rel[tm + aoff], -acc[bi][aj][v]);
+          // rel is defined for rows a >= the first row of b's target group
+          const int ac = min(max(aoff, bc), m - 1);
+          rl[bi][v][aj] = rel[tm[bi][v] + ac];
+        }
+      }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) keep(rl[bi][v][aj]);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) {
+          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
+          if (boff < m && aoff < m && aoff >= boff)
+            atomic_add(arena + tc[bi][v] + rl[bi][v][aj], -acc[bi][aj][v]);
         }
       }
   }

@@ -140,7 +140,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
         int c0 = st * kNB, bs = std::min(kNB, si.w - c0);
         int64_t rows_below = si.r - c0 - bs;
         p.potrf_tasks.push_back({I, st, 0, 0});
-        for (int64_t rt = 0; rt < ceil_div(rows_below, kTile); ++rt)
+        for (int64_t rt = 0; rt < ceil_div(rows_below, kTrsmRows); ++rt)
           p.trsm_tasks.push_back({I, st, (int32_t)rt, 0});
         p.diag_flops += (double)bs * bs * bs / 3.0 + (double)bs * bs * rows_below;
         if (st < nb - 1) {

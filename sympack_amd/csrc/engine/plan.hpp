@@ -21,6 +21,7 @@ namespace spk {
 
 constexpr int kNB = 64;    // diagonal block width of the blocked panel factorization
 constexpr int kTile = 64;  // update tile edge (rows x cols)
+constexpr int kTrsmRows = 256;  // panel rows per TRSM workgroup
 
 struct SnInfo {            // mirrored on the device (16-byte aligned)
   int64_t off;             // panel offset in the arena (elements)

@@ -248,6 +248,15 @@ class symPACKMatrix2D:
     def reset_kernel_stats(self):
         self._lib.spk_reset_kernel_stats(self._h)
 
+    def launch_trace(self, max_records=100000):
+        k = np.zeros(max_records, np.int32)
+        g = np.zeros(max_records, np.int64)
+        ms = np.zeros(max_records, np.float64)
+        cnt = _lib.check(self._lib.spk_get_launch_trace(self._h, _ptr(k, ctypes.c_int32),
+                                                        _ptr(g, ctypes.c_int64),
+                                                        _ptr(ms, ctypes.c_double), max_records), "trace")
+        return k[:cnt], g[:cnt], ms[:cnt]
+
     def set_profile(self, on):
         _lib.check(self._lib.spk_set_profile(self._h, 1 if on else 0), "set_profile")
 
