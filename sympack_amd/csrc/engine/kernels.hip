@@ -173,11 +173,50 @@ __global__ __launch_bounds__(64) void k_potrf(const SnInfo* __restrict__ sn,
 // ---------------------------------------------------------------------------
 constexpr int TRSM_ROWS = 256;
 
+// Body of the TRSM for a diagonal block of width bs <= W (W = 16, 32, 64):
+// the unrolled loops only run over W columns.
+template <typename T, bool LDL, int W>
+__device__ __forceinline__ void trsm_rows(T* __restrict__ P, int64_t ld, int c0, int bs, int R0,
+                                          int nrows_panel, T (*Lc)[NB], const T* invd) {
+  const int tid = threadIdx.x;
+  // rows past the panel work on a clamped (valid) row and do not store
+  const bool active = R0 + tid < nrows_panel;
+  const int row = min(R0 + tid, nrows_panel - 1);
+  // columns >= bs read column bs-1 (a valid address) and are zeroed; Lc is
+  // zero on and above its diagonal and invd is zero beyond bs, so the
+  // substitution needs no per-element guards.
+  T x[W];
+  {
+    const T* prow = P + (int64_t)c0 * ld + row;
+#pragma unroll
+    for (int j = 0; j < W; ++j) x[j] = prow[(int64_t)min(j, bs - 1) * ld];
+  }
+#pragma unroll
+  for (int j = 0; j < W; ++j) keep(x[j]);
+#pragma unroll
+  for (int j = 0; j < W; ++j) x[j] = (j < bs) ? x[j] : T(0);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const T xj = LDL ? x[j] : x[j] * invd[j];
+#pragma unroll
+    for (int k = j + 1; k < W; ++k) x[k] -= xj * Lc[j][k];
+  }
+  if (!active) return;
+  // a fresh row pointer: the load addresses must not stay live until here
+  T* prow = P + (int64_t)c0 * ld + row;
+  asm volatile("" : "+v"(prow));
+  // unconditional stores in descending column order: columns >= bs write
+  // zeros to column bs-1, which its own (correct) value then overwrites; a
+  // guard per column would split the code into many blocks and spill
+#pragma unroll
+  for (int j = W - 1; j >= 0; --j) prow[(int64_t)min(j, bs - 1) * ld] = x[j] * invd[j];
+}
+
 template <typename T, bool LDL>
 __global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
                                               const Task4* __restrict__ tasks,
                                               T* __restrict__ arena) {
-  __shared__ __attribute__((aligned(16))) T Lc[NB][NB];  // Lc[j][k] = L(k, j)
+  __shared__ __attribute__((aligned(16))) T Lc[NB][NB];  // Lc[j][k] = L(k, j), k > j
   __shared__ T invd[NB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
@@ -194,38 +233,12 @@ __global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
     if (k == j) invd[j] = (j < bs) ? T(1) / v : T(0);
   }
   __syncthreads();
-  // rows past the panel work on a clamped (valid) row and do not store; all
-  // threads stay for the barriers below
-  const bool active = R0 + tid < s.r;
-  const int row = min(R0 + tid, s.r - 1);
-  // columns >= bs read column bs-1 (a valid address) and are zeroed; Lc is
-  // zero on and above its diagonal and invd is zero beyond bs, so the
-  // substitution needs no per-element guards.
-  T x[NB];
-  {
-    const T* prow = P + (int64_t)c0 * ld + row;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) x[j] = prow[(int64_t)min(j, bs - 1) * ld];
-  }
-#pragma unroll
-  for (int j = 0; j < NB; ++j) keep(x[j]);
-#pragma unroll
-  for (int j = 0; j < NB; ++j) x[j] = (j < bs) ? x[j] : T(0);
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    const T xj = LDL ? x[j] : x[j] * invd[j];
-#pragma unroll
-    for (int k = j + 1; k < NB; ++k) x[k] -= xj * Lc[j][k];
-  }
-  if (!active) return;
-  // a fresh row pointer: the load addresses must not stay live until here
-  T* prow = P + (int64_t)c0 * ld + row;
-  asm volatile("" : "+v"(prow));
-  // unconditional stores in descending column order: columns >= bs write
-  // zeros to column bs-1, which its own (correct) value then overwrites; a
-  // guard per column would split the code into 64 blocks and spill
-#pragma unroll
-  for (int j = NB - 1; j >= 0; --j) prow[(int64_t)min(j, bs - 1) * ld] = x[j] * invd[j];
+  if (bs <= 16)
+    trsm_rows<T, LDL, 16>(P, ld, c0, bs, R0, s.r, Lc, invd);
+  else if (bs <= 32)
+    trsm_rows<T, LDL, 32>(P, ld, c0, bs, R0, s.r, Lc, invd);
+  else
+    trsm_rows<T, LDL, 64>(P, ld, c0, bs, R0, s.r, Lc, invd);
 }
 
 // ---------------------------------------------------------------------------
@@ -430,14 +443,68 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 // ---------------------------------------------------------------------------
 constexpr int JC = 64;  // right-hand sides per LDS chunk
 
+__device__ __forceinline__ double readlane_t(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float readlane_t(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Triangular solve of one diagonal block (bs <= 64) for right-hand sides
+// handled one per wave: lane i holds x_i, the pivot value is broadcast with
+// readlane and every lane applies one fused multiply-add per step.
+// Forward: L x = z; backward: L^T x = z.  Lk[j][k] = L(c0+k, c0+j) for k > j
+// (zero elsewhere); invd = 1/L_jj (LL) or 1 (LDL, unit diagonal).
+template <typename T, bool BWD>
+__device__ __forceinline__ void block_solve_waves(T* __restrict__ X, int64_t rowbase, int nrhs,
+                                                  int bs, T (*Lk)[NB + 1], const T* invd) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = min(lane, bs - 1);
+  for (int jj = wave; jj < nrhs; jj += NT / 64) {
+    T* col = X + rowbase * nrhs + jj;
+    T xi = col[(int64_t)lr * nrhs];
+    xi = (lane < bs) ? xi : T(0);
+    if (!BWD) {
+      for (int j = 0; j < bs; ++j) {
+        const T xj = readlane_t(xi, j) * invd[j];
+        xi = (lane == j) ? xj : xi - Lk[j][lane] * xj;
+      }
+    } else {
+      for (int j = bs - 1; j >= 0; --j) {
+        const T xj = readlane_t(xi, j) * invd[j];
+        xi = (lane == j) ? xj : xi - Lk[lane][j] * xj;
+      }
+    }
+    if (lane < bs) col[(int64_t)lane * nrhs] = xi;
+  }
+}
+
+// Stage diagonal block c0 of a panel in LDS: Lk[j][k] = L(c0+k, c0+j) for
+// k > j (zero elsewhere) and invd[j] = 1/L(c0+j,c0+j) (LL) or 1 (LDL).
+template <typename T, bool LDL>
+__device__ __forceinline__ void stage_diag_block(const T* __restrict__ P, int64_t ld, int c0, int bs,
+                                                 T (*Lk)[NB + 1], T* invd) {
+  for (int idx = threadIdx.x; idx < NB * NB; idx += NT) {
+    const int k = idx % NB, j = idx / NB;
+    const T v = P[(int64_t)(c0 + min(j, bs - 1)) * ld + c0 + min(k, bs - 1)];
+    Lk[j][k] = (k < bs && j < bs && k > j) ? v : T(0);
+    if (k == j) invd[j] = (j < bs) ? (LDL ? T(1) : T(1) / v) : T(0);
+  }
+}
+
+// Forward diagonal solve of a supernode: X_I = L_II^{-1} Y_I, blocked by 64
+// columns; task {sn}.  Threads own right-hand sides for the block solves and
+// (row, rhs) pairs for the updates of the supernode's remaining rows.
 template <typename T, bool LDL>
 __global__ __launch_bounds__(NT) void k_sdiag_fwd(const SnInfo* __restrict__ sn,
                                                    const Task4* __restrict__ tasks,
                                                    const T* __restrict__ arena,
                                                    const T* __restrict__ Y, T* __restrict__ X,
                                                    int nrhs) {
-  __shared__ T Ld[NB][NB + 1];
-  __shared__ T Z[NB][JC + 1];
+  __shared__ T Lk[NB][NB + 1];
   __shared__ T invd[NB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
@@ -449,48 +516,30 @@ __global__ __launch_bounds__(NT) void k_sdiag_fwd(const SnInfo* __restrict__ sn,
   __syncthreads();
   for (int c0 = 0; c0 < s.w; c0 += NB) {
     const int bs = min(NB, s.w - c0);
-    for (int idx = tid; idx < bs * bs; idx += NT) {
-      int i = idx % bs, j = idx / bs;
-      if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
+    stage_diag_block<T, LDL>(P, ld, c0, bs, Lk, invd);
+    __syncthreads();
+    block_solve_waves<T, false>(X, (int64_t)s.fc + c0, nrhs, bs, Lk, invd);
+    __syncthreads();
+    // rows of the supernode below this block: X_i -= L(i, block) x_block;
+    // fixed-length unrolled dot products (clamped columns, zero weights
+    // beyond bs) keep all loads of a row in flight.
+    const int below = s.w - c0 - bs;
+    for (int idx = tid; idx < below * nrhs; idx += NT) {
+      const int i = c0 + bs + idx / nrhs, jj = idx % nrhs;
+      const T* lrow = P + (int64_t)c0 * ld + i;
+      const T* xb = X + base + (int64_t)c0 * nrhs + jj;
+      T lv[NB], xv[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        lv[k] = lrow[(int64_t)min(k, bs - 1) * ld];
+        xv[k] = xb[(int64_t)min(k, bs - 1) * nrhs];
+      }
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < NB; ++k) acc += (k < bs) ? lv[k] * xv[k] : T(0);
+      X[base + (int64_t)i * nrhs + jj] -= acc;
     }
     __syncthreads();
-    if (tid < bs) invd[tid] = LDL ? T(1) : T(1) / Ld[tid][tid];
-    for (int j0 = 0; j0 < nrhs; j0 += JC) {
-      const int jc = min(JC, nrhs - j0);
-      for (int idx = tid; idx < bs * jc; idx += NT) {
-        int i = idx / jc, jj = idx % jc;
-        Z[i][jj] = X[base + (int64_t)(c0 + i) * nrhs + j0 + jj];
-      }
-      __syncthreads();
-      for (int k = 0; k < bs; ++k) {
-        const int cnt = (bs - k - 1) * jc;
-        for (int idx = tid; idx < cnt; idx += NT) {
-          int i = k + 1 + idx / jc, jj = idx % jc;
-          Z[i][jj] -= Ld[i][k] * (Z[k][jj] * invd[k]);
-        }
-        __syncthreads();
-      }
-      for (int idx = tid; idx < bs * jc; idx += NT) {
-        int i = idx / jc, jj = idx % jc;
-        T v = Z[i][jj] * invd[i];
-        X[base + (int64_t)(c0 + i) * nrhs + j0 + jj] = v;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < bs * jc; idx += NT) {
-        int i = idx / jc, jj = idx % jc;
-        Z[i][jj] *= invd[i];
-      }
-      __syncthreads();
-      // rows of this supernode below the block
-      const int below = s.w - c0 - bs;
-      for (int idx = tid; idx < below * jc; idx += NT) {
-        int i = c0 + bs + idx / jc, jj = idx % jc;
-        T acc = T(0);
-        for (int k = 0; k < bs; ++k) acc += P[(int64_t)(c0 + k) * ld + i] * Z[k][jj];
-        X[base + (int64_t)i * nrhs + j0 + jj] -= acc;
-      }
-      __syncthreads();
-    }
   }
 }
 
@@ -591,13 +640,14 @@ __global__ __launch_bounds__(NT) void k_scale_d(const SnInfo* __restrict__ sn,
   }
 }
 
+// Backward diagonal solve of a supernode: X_I = L_II^{-T} X_I, blocks from
+// the last one; task {sn}.
 template <typename T, bool LDL>
 __global__ __launch_bounds__(NT) void k_sdiag_bwd(const SnInfo* __restrict__ sn,
                                                    const Task4* __restrict__ tasks,
                                                    const T* __restrict__ arena,
                                                    T* __restrict__ X, int nrhs) {
-  __shared__ T Ld[NB][NB + 1];
-  __shared__ T Z[NB][JC + 1];
+  __shared__ T Lk[NB][NB + 1];
   __shared__ T invd[NB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
@@ -609,39 +659,30 @@ __global__ __launch_bounds__(NT) void k_sdiag_bwd(const SnInfo* __restrict__ sn,
   for (int blk = nb - 1; blk >= 0; --blk) {
     const int c0 = blk * NB;
     const int bs = min(NB, s.w - c0);
-    __syncthreads();
-    for (int idx = tid; idx < bs * bs; idx += NT) {
-      int i = idx % bs, j = idx / bs;
-      if (i >= j) Ld[i][j] = P[(int64_t)(c0 + j) * ld + c0 + i];
-    }
-    __syncthreads();
-    if (tid < bs) invd[tid] = LDL ? T(1) : T(1) / Ld[tid][tid];
-    for (int j0 = 0; j0 < nrhs; j0 += JC) {
-      const int jc = min(JC, nrhs - j0);
-      // z_blk -= L[rows below the block inside the supernode, blk]^T x_below
-      const int below = s.w - c0 - bs;
-      for (int idx = tid; idx < bs * jc; idx += NT) {
-        int kq = idx / jc, jj = idx % jc;
-        T acc = T(0);
-        for (int i = 0; i < below; ++i)
-          acc += P[(int64_t)(c0 + kq) * ld + c0 + bs + i] *
-                 X[base + (int64_t)(c0 + bs + i) * nrhs + j0 + jj];
-        Z[kq][jj] = X[base + (int64_t)(c0 + kq) * nrhs + j0 + jj] - acc;
-      }
-      __syncthreads();
-      for (int k = bs - 1; k >= 0; --k) {
-        for (int idx = tid; idx < k * jc; idx += NT) {
-          int i = idx / jc, jj = idx % jc;
-          Z[i][jj] -= Ld[k][i] * (Z[k][jj] * invd[k]);
+    // X_block -= L(rows below the block inside the supernode, block)^T x_below
+    const int below = s.w - c0 - bs;
+    for (int idx = tid; idx < bs * nrhs; idx += NT) {
+      const int k = idx / nrhs, jj = idx % nrhs;
+      const T* lcol = P + (int64_t)(c0 + k) * ld + c0 + bs;
+      const T* xb = X + base + (int64_t)(c0 + bs) * nrhs + jj;
+      T acc = T(0);
+      for (int i0 = 0; i0 < below; i0 += NB) {
+        const int cnt = min(NB, below - i0);
+        T lv[NB], xv[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          lv[i] = lcol[i0 + min(i, cnt - 1)];
+          xv[i] = xb[(int64_t)(i0 + min(i, cnt - 1)) * nrhs];
         }
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) acc += (i < cnt) ? lv[i] * xv[i] : T(0);
       }
-      for (int idx = tid; idx < bs * jc; idx += NT) {
-        int i = idx / jc, jj = idx % jc;
-        X[base + (int64_t)(c0 + i) * nrhs + j0 + jj] = Z[i][jj] * invd[i];
-      }
-      __syncthreads();
+      X[base + (int64_t)(c0 + k) * nrhs + jj] -= acc;
     }
+    stage_diag_block<T, LDL>(P, ld, c0, bs, Lk, invd);
+    __syncthreads();
+    block_solve_waves<T, true>(X, (int64_t)s.fc + c0, nrhs, bs, Lk, invd);
+    __syncthreads();
   }
 }
 
