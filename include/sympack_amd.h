@@ -85,6 +85,8 @@ int spk_reset_kernel_stats(spk_handle h);
 /* Switch per-launch event timing on/off after spk_init (no reference
  * counterpart; the reference's timing is the SPROFILE build, timer.hpp:56-85). */
 int spk_set_profile(spk_handle h, int on);
+/* As spk_set_profile, timing only the kernel kinds whose bit is set in mask. */
+int spk_set_profile_mask(spk_handle h, int on, unsigned mask);
 /* Per-launch records of the profiled calls since the last reset: kind, grid
  * size (workgroups) and event time; returns the number of records (<= max). */
 int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max);
@@ -142,6 +144,13 @@ int spk_get_cells(spk_handle h, int32_t* cell_K, int32_t* cell_I, int32_t* cell_
  * by supernode columns; LDL panels hold D on the diagonal. */
 int spk_get_panel_dims(spk_handle h, int32_t I, int32_t* nrows, int32_t* ncols);
 int spk_get_factor_panel(spk_handle h, int32_t I, double* out);
+
+/* The ordering routine alone, as symPACK::Ordering::{MMD,RCM,AMD,NDBOX,NDGRID}
+ * computes it (src/Ordering.cpp:103-525) before the elimination-tree
+ * postorder: graph = lower CSC pattern expanded, diagonal dropped.
+ * perm[new-1] = old, invp[old-1] = new (1-based). */
+int spk_compute_ordering(int ordering, int32_t n, const int64_t* colptr, const int32_t* rowind,
+                         int baseval, int32_t* perm, int32_t* invp);
 
 /* Rutherford-Boeing / Harwell-Boeing I/O (ReadHB_PARA, utility.hpp:1274-1482). */
 int spk_read_rb(const char* filename, int32_t* n, int64_t* nnz);

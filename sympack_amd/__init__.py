@@ -6,8 +6,8 @@ interface and provides the benchmark matrix generators.
 """
 from . import matrices  # noqa: F401
 from ._lib import NotHPDError, SymPACKError, load  # noqa: F401
-from .matrix import (DistSparseMatrix, RelaxationParameters, symPACKMatrix2D,  # noqa: F401
-                     symPACKOptions)
+from .matrix import (DistSparseMatrix, RelaxationParameters, compute_ordering,  # noqa: F401
+                     symPACKMatrix2D, symPACKOptions)
 
 __all__ = ["DistSparseMatrix", "RelaxationParameters", "symPACKMatrix2D", "symPACKOptions",
            "NotHPDError", "SymPACKError", "load", "matrices"]

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("spk_get_kernel_stats", _I, [_P, _I, _PI64, _PD]),
     ("spk_reset_kernel_stats", _I, [_P]),
     ("spk_set_profile", _I, [_P, _I]),
+    ("spk_set_profile_mask", _I, [_P, _I, ctypes.c_uint]),
     ("spk_get_launch_trace", _I, [_P, _PI32, _PI64, _PD, _I]),
     ("spk_get_supernodal_partition", _I, [_P, _PI32]),
     ("spk_get_sup_membership", _I, [_P, _PI32]),
@@ -100,6 +101,7 @@ SIGNATURES = [
     ("spk_get_cells", _I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32, _PI32]),
     ("spk_get_panel_dims", _I, [_P, _I32, _PI32, _PI32]),
     ("spk_get_factor_panel", _I, [_P, _I32, _PD]),
+    ("spk_compute_ordering", _I, [_I, _I32, _PI64, _PI32, _I, _PI32, _PI32]),
     ("spk_read_rb", _I, [ctypes.c_char_p, _PI32, _PI64]),
     ("spk_read_rb_data", _I, [ctypes.c_char_p, _PI64, _PI32, _PD]),
     ("spk_write_rb", _I, [ctypes.c_char_p, _I32, _PI64, _PI32, _PD, ctypes.c_char_p]),

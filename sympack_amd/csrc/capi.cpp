@@ -330,6 +330,12 @@ int spk_set_profile(spk_handle h, int on) {
   return 0;
 }
 
+int spk_set_profile_mask(spk_handle h, int on, unsigned mask) {
+  if (!h) return fail("null handle");
+  h->engine.set_profile(on != 0, mask);
+  return 0;
+}
+
 int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max) {
   if (!h) return fail("null handle");
   const auto& t = h->engine.trace;
@@ -446,6 +452,34 @@ int spk_get_factor_panel(spk_handle h, int32_t I, double* out) {
   return guarded([&]() {
     if (!h->engine.factored()) throw std::logic_error("no factor available");
     h->engine.get_panel(I - 1, out);
+    return 0;
+  });
+}
+
+int spk_compute_ordering(int ordering, int32_t n, const int64_t* colptr, const int32_t* rowind,
+                         int baseval, int32_t* perm, int32_t* invp) {
+  if (!colptr || !rowind || !perm || !invp) return fail("null argument");
+  if (n <= 0) return fail("matrix dimension must be positive");
+  return guarded([&]() {
+    spk::Ordering ord;
+    if (ordering == SPK_ORDER_NDBOX) {
+      spk::order_ndbox(n, ord);
+    } else if (ordering == SPK_ORDER_NDGRID) {
+      spk::order_ndgrid(n, ord);
+    } else {
+      spk::Graph g = spk::expand_symmetric(n, colptr, rowind, baseval);
+      spk::set_keep_diag(g, false);
+      if (ordering == SPK_ORDER_MMD)
+        spk::order_mmd(g, ord);
+      else if (ordering == SPK_ORDER_RCM)
+        spk::order_rcm(g, ord);
+      else if (ordering == SPK_ORDER_AMD)
+        spk::order_amd(g, ord);
+      else
+        throw std::logic_error("spk_compute_ordering: unsupported ordering code");
+    }
+    std::memcpy(perm, ord.perm.data(), n * sizeof(int32_t));
+    std::memcpy(invp, ord.invp.data(), n * sizeof(int32_t));
     return 0;
   });
 }

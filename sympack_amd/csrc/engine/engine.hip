@@ -216,7 +216,7 @@ void Engine::assemble(T* arena) {
 
 void Engine::launch_begin(int kind, int64_t grid) {
   kind_stats[kind].launches++;
-  if (!opt_.profile) return;
+  if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
   ev_grid_.push_back(grid);
   while (ev_pool_.size() < ev_next_ + 2) {
     hipEvent_t e;
@@ -230,8 +230,7 @@ void Engine::launch_begin(int kind, int64_t grid) {
 }
 
 void Engine::launch_end(int kind) {
-  (void)kind;
-  if (!opt_.profile) return;
+  if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
   HIP_OK(hipEventRecord(ev_open_.back().second.second, stream_));
 }
 

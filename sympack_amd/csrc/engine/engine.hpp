@@ -46,7 +46,10 @@ class Engine {
 
   void init(const EngineOptions& o);
   void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
-  void set_profile(bool on) { opt_.profile = on ? 1 : 0; }
+  void set_profile(bool on, unsigned mask = 0xffu) {
+    opt_.profile = on ? 1 : 0;
+    profile_mask_ = mask;
+  }
   void symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval);
   void distribute(const double* nzval, bool on_device);
   int factorize();
@@ -125,6 +128,7 @@ class Engine {
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
   std::vector<int64_t> ev_grid_;
+  unsigned profile_mask_ = 0xffu;  // kinds whose launches get events
   size_t ev_next_ = 0;
   void collect_events();
 };

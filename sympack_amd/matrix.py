@@ -257,8 +257,15 @@ class symPACKMatrix2D:
                                                         _ptr(ms, ctypes.c_double), max_records), "trace")
         return k[:cnt], g[:cnt], ms[:cnt]
 
-    def set_profile(self, on):
-        _lib.check(self._lib.spk_set_profile(self._h, 1 if on else 0), "set_profile")
+    def set_profile(self, on, kinds=None):
+        """Time kernel launches with HIP events (all kinds, or the given kinds)."""
+        if kinds is None:
+            _lib.check(self._lib.spk_set_profile(self._h, 1 if on else 0), "set_profile")
+        else:
+            mask = 0
+            for k in kinds:
+                mask |= 1 << k
+            _lib.check(self._lib.spk_set_profile_mask(self._h, 1 if on else 0, mask), "set_profile")
 
     def TotalSupernodeCnt(self):
         return self.stats()["nsuper"]
@@ -318,3 +325,16 @@ class symPACKMatrix2D:
         out = np.zeros(r.value * w.value, np.float64)
         _lib.check(self._lib.spk_get_factor_panel(self._h, I, _ptr(out, ctypes.c_double)), "panel")
         return out.reshape(w.value, r.value).T.copy()
+
+
+def compute_ordering(ordering, A):
+    """The ordering routine alone (symPACK::Ordering, src/Ordering.cpp:103-525):
+    returns (perm, invp), 1-based, before the elimination-tree postorder."""
+    lib = _lib.load()
+    perm = np.zeros(A.size, np.int32)
+    invp = np.zeros(A.size, np.int32)
+    _lib.check(lib.spk_compute_ordering(_lib.ORDERINGS[ordering], A.size, _ptr(A.colptr, ctypes.c_int64),
+                                        _ptr(A.rowind, ctypes.c_int32), A.baseval,
+                                        _ptr(perm, ctypes.c_int32), _ptr(invp, ctypes.c_int32)),
+               "ordering")
+    return perm, invp

@@ -1,0 +1,92 @@
+"""C ABI surface (every declared symbol is exported and bound), the option
+parser, Rutherford-Boeing I/O and the matrix generators (CPU only)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sympack_amd as sp
+from sympack_amd import _lib
+from sympack_amd import matrices as M
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "sympack_amd.h")
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(spk_\w+|symPACK_\w+)\s*\(", text))
+    lib = _lib.load()
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert names, "no declarations parsed"
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in bound, n
+
+
+def test_rank_and_init():
+    lib = _lib.load()
+    import ctypes
+    r = ctypes.c_int(-1)
+    assert lib.symPACK_Init(None, None) == 0
+    assert lib.symPACK_Rank(ctypes.byref(r)) == 0 and r.value == 0
+    assert lib.symPACK_Finalize() == 0
+
+
+def test_options_from_args_like_run_sympack2D():
+    o = sp.symPACKOptions.from_args(["-in", "x.rb", "-ordering", "NDBOX", "-fact", "LDL", "-relax", "4", "16",
+                                     "32", "-b", "100", "-precision", "FP32"])
+    assert o.orderingStr == "NDBOX" and o.decomposition == "LDL"
+    assert (o.relax.nrelax0, o.relax.nrelax1, o.relax.nrelax2, o.relax.maxSize) == (4, 16, 32, 100)
+    assert o.precision == "FP32_IR"
+    o = sp.symPACKOptions.from_args(["-relax", "off"])
+    assert o.relax.nrelax0 == 0
+    o = sp.symPACKOptions.from_args([])
+    assert o.orderingStr == "MMD" and o.relax.maxSize == 0
+
+
+def test_rb_round_trip(tmp_path):
+    n, cp, ri, v = M.diffusion_3d_7pt(5, seed=3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    f = str(tmp_path / "a.rb")
+    A.write_rb(f)
+    B = sp.DistSparseMatrix.from_rb(f)
+    assert B.size == n and B.baseval == 1
+    assert np.array_equal(B.colptr - 1, cp)
+    assert np.array_equal(B.rowind - 1, ri)
+    assert np.array_equal(B.nzval, v)  # 17 significant digits: exact
+
+
+def test_rb_reads_reference_style_file(tmp_path):
+    # 3x3 lower triangle, fixed-width records as ReadHB_PARA expects
+    f = tmp_path / "small.rb"
+    f.write_text(
+        "small test matrix                                                       KEY     \n"
+        "             3             1             1             1             0\n"
+        "rsa                        3             3             5             0\n"
+        "(4I5)           (5I5)           (5E15.6)            \n"
+        "    1    3    5    6\n"
+        "    3    1    2    3    3\n"
+        "   4.000000E+00  -1.000000E+00   4.000000E+00  -1.000000E+00   4.000000E+00\n")
+    A = sp.DistSparseMatrix.from_rb(str(f))
+    assert A.size == 3
+    assert list(A.colptr) == [1, 3, 5, 6]
+    assert list(A.rowind) == [1, 3, 2, 3, 3]  # rows sorted per column
+    assert list(A.nzval) == [-1.0, 4.0, 4.0, -1.0, 4.0]
+
+
+def test_generators_sizes_and_symmetry():
+    n, cp, ri, v = M.laplacian_2d_5pt(64)
+    assert n == 4096 and cp[-1] == 12160
+    n, cp, ri, v = M.diffusion_3d_7pt(48)
+    assert n == 110592 and cp[-1] == 435456
+    n, cp, ri, v = M.shifted_laplacian_3d_7pt(80)
+    assert n == 512000 and cp[-1] == 2028800
+    n, cp, ri, v = M.stencil_3d_27pt(12)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    F = A.to_scipy_full()
+    assert abs(F - F.T).max() == 0
+    d = F.diagonal()
+    off = np.asarray(abs(F).sum(axis=1)).ravel() - d
+    assert np.all(d > off)  # strictly diagonally dominant
