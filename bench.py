@@ -35,6 +35,119 @@ def build_problem(k):
     return sp.DistSparseMatrix(n, cp, ri, v)
 
 
+def precision_leg(k, nrhs, reps):
+    """FP32 factor + FP64 iterative refinement against the FP64 factor on the
+    27-point 60^3 stencil: time to solution (factor + solve of nrhs Gaussian
+    right-hand sides, inputs resident in HBM), backward errors, refinement
+    steps and the forward difference between the two solutions."""
+    import numpy as np
+    import torch
+
+    import sympack_amd as sp
+    from oracle import numeric as onum
+
+    n, cp, ri, v = sp.matrices.stencil_3d_27pt(k, seed=0)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    dev = torch.device("cuda:0")
+    vals = torch.from_numpy(A.nzval).to(dev)
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((n, nrhs))
+    b = torch.from_numpy(np.ascontiguousarray(B.ravel(order="F"))).to(dev)
+    out = {}
+    X = {}
+    for prec in ("FP64", "FP32_IR"):
+        rp = sp.RelaxationParameters()
+        rp.SetMaxSize(150)
+        M = sp.symPACKMatrix2D()
+        M.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision=prec, ir_tol=1e-12))
+        M.SymbolicFactorization(A)
+        torch.cuda.synchronize()
+        M.DistributeMatrixDevice(vals.data_ptr())
+        M.Factorize()
+        M.SolveDevice(b.data_ptr(), nrhs)
+        torch.cuda.synchronize()
+        tf, ts = [], []
+        for _ in range(reps):
+            a = time.perf_counter()
+            M.Factorize()
+            c = time.perf_counter()
+            M.SolveDevice(b.data_ptr(), nrhs)
+            torch.cuda.synchronize()
+            d = time.perf_counter()
+            tf.append((c - a) * 1e3)
+            ts.append((d - c) * 1e3)
+        x = torch.empty(n * nrhs, dtype=torch.float64, device=dev)
+        M.GetSolutionDevice(x.data_ptr(), nrhs)
+        X[prec] = x.cpu().numpy().reshape(nrhs, n).T
+        st = M.stats()
+        key = "fp64" if prec == "FP64" else "fp32_ir"
+        out[key + "_ms"] = round(float(np.median(np.add(tf, ts))), 3)
+        out[key + "_factor_ms"] = round(float(np.median(tf)), 3)
+        out[key + "_solve_ms"] = round(float(np.median(ts)), 3)
+        out[key + "_backward_error"] = onum.backward_error(A, X[prec], B)
+        if prec == "FP32_IR":
+            out["ir_iterations"] = st["ir_iterations"]
+        out["flops_sum_cc2"] = st["flops"]
+        del M
+    out["speedup_fp32_ir_vs_fp64"] = round(out["fp64_ms"] / out["fp32_ir_ms"], 3)
+    out["forward_diff_fp32_ir_vs_fp64"] = float(np.linalg.norm(X["FP32_IR"] - X["FP64"]) /
+                                                np.linalg.norm(X["FP64"]))
+    out["workload"] = ("3D 27-point SPD stencil %d^3 (n=%d), NDBOX, nrhs=%d Gaussian, median of %d "
+                       "factor+solve runs" % (k, n, nrhs, reps))
+    return out
+
+
+def serving_leg(n_requests, rate, precision, slots, seed=0):
+    """Request mix through the serving layer: new factors of 2D/3D
+    random-coefficient Laplacians (n log-uniform in [2e3, 2e5]) and follow-up
+    solves (nrhs 1..32) on resident factors, Poisson arrivals; returns the
+    latency percentiles and throughput."""
+    import numpy as np
+
+    import sympack_amd as sp
+    from sympack_amd.serving import SolverService, make_request_stream
+
+    stream = make_request_stream(n_requests, rate, seed=seed)
+    mats, rhs = {}, []
+    for t, kind, key, p in stream:
+        if kind == "factor":
+            dim, k, s = p
+            n, cp, ri, v = sp.matrices.random_laplacian("2d" if dim == 2 else "3d", k, s)
+            mats[key] = (dim, sp.DistSparseMatrix(n, cp, ri, v))
+    for t, kind, key, p in stream:
+        if kind == "solve":
+            rhs.append(np.random.default_rng(p[1]).standard_normal((mats[key][1].size, p[0])))
+
+    def opts(dim):
+        rp = sp.RelaxationParameters()
+        rp.SetMaxSize(150)
+        return sp.symPACKOptions(ordering="NDGRID" if dim == 2 else "NDBOX", relax=rp, precision=precision)
+
+    futs = []
+    with SolverService(slots=slots) as svc:
+        t0 = time.perf_counter()
+        j = 0
+        for t, kind, key, p in stream:
+            dt = t0 + t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            if kind == "factor":
+                futs.append(svc.submit_factor(key, mats[key][1], opts(mats[key][0])))
+            else:
+                futs.append(svc.submit_solve(key, rhs[j]))
+                j += 1
+        for f in futs:
+            f.result()
+        rep = svc.report()
+    nf = sum(1 for r in stream if r[1] == "factor")
+    rep["requests"] = n_requests
+    rep["factor_requests"] = nf
+    rep["arrival_rate_per_s"] = rate
+    rep["slots"] = slots
+    rep["precision"] = precision
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,6 +156,13 @@ def main():
     ap.add_argument("--k", type=int, default=48, help="grid edge (48 -> n=110592)")
     ap.add_argument("--solves", type=int, default=50, help="extra solve-only calls for latency")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32+IR vs FP64 comparison on the 60^3 27-point stencil")
+    ap.add_argument("--precision-k", type=int, default=60)
+    ap.add_argument("--serving", type=int, default=0, metavar="N",
+                    help="also run N requests of the serving mix (FP64 and FP32+IR)")
+    ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the factorization on one GPU only; "
@@ -138,6 +258,12 @@ def main():
                          "by oracle/supernodal.c on 1 host core: %.1f s; one solve %.3f s" %
                          (args.k, t_cf, t_cs)}
 
+    prec = None if args.no_precision else precision_leg(args.precision_k, 16, 3)
+    serving = None
+    if args.serving:
+        serving = [serving_leg(args.serving, args.serving_rate, p, args.serving_slots)
+                   for p in ("FP64", "FP32_IR")]
+
     out = {
         "metric": "factor GFLOP/s; solves/s + p99 latency, FP32+IR vs FP64 at bwd err <=1e-12",
         "value": round(value, 2),
@@ -178,6 +304,8 @@ def main():
                      "flops_per_launch": schur_flops_per_launch,
                      "avg_launch_ms": schur_avg_ms},
         "cpu_baseline": cpu,
+        "precision_fp32_ir_vs_fp64": prec,
+        "serving": serving,
     }
     print(json.dumps(out))
 

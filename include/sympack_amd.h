@@ -87,6 +87,17 @@ int spk_reset_kernel_stats(spk_handle h);
 int spk_set_profile(spk_handle h, int on);
 /* As spk_set_profile, timing only the kernel kinds whose bit is set in mask. */
 int spk_set_profile_mask(spk_handle h, int on, unsigned mask);
+/* HBM footprint of the handle after spk_distribute_matrix (plan, maps,
+ * values, factor) plus solve buffers for nrhs right-hand sides; valid after
+ * spk_symbolic_factorization.  Used for admission by memory budget. */
+int spk_device_bytes(spk_handle h, int nrhs, int64_t* bytes);
+/* Free the handle's device memory, keeping the symbolic analysis; the values
+ * must be distributed again before the next spk_factorize. */
+int spk_release_device(spk_handle h);
+/* Replay factor and solve launch sequences from captured HIP graphs (on = 1,
+ * the default) or launch each kernel from the host (on = 0).  No reference
+ * counterpart (the reference has no device launch sequences). */
+int spk_set_graphs(spk_handle h, int on);
 /* Per-launch records of the profiled calls since the last reset: kind, grid
  * size (workgroups) and event time; returns the number of records (<= max). */
 int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max);

@@ -89,6 +89,9 @@ SIGNATURES = [
     ("spk_reset_kernel_stats", _I, [_P]),
     ("spk_set_profile", _I, [_P, _I]),
     ("spk_set_profile_mask", _I, [_P, _I, ctypes.c_uint]),
+    ("spk_set_graphs", _I, [_P, _I]),
+    ("spk_device_bytes", _I, [_P, _I, _PI64]),
+    ("spk_release_device", _I, [_P]),
     ("spk_get_launch_trace", _I, [_P, _PI32, _PI64, _PD, _I]),
     ("spk_get_supernodal_partition", _I, [_P, _PI32]),
     ("spk_get_sup_membership", _I, [_P, _PI32]),

@@ -336,6 +336,28 @@ int spk_set_profile_mask(spk_handle h, int on, unsigned mask) {
   return 0;
 }
 
+int spk_device_bytes(spk_handle h, int nrhs, int64_t* bytes) {
+  if (!h || !bytes) return fail("null argument");
+  return guarded([&]() {
+    *bytes = h->engine.device_bytes(nrhs);
+    return 0;
+  });
+}
+
+int spk_release_device(spk_handle h) {
+  if (!h) return fail("null handle");
+  return guarded([&]() {
+    h->engine.release_device();
+    return 0;
+  });
+}
+
+int spk_set_graphs(spk_handle h, int on) {
+  if (!h) return fail("null handle");
+  h->engine.set_graphs(on != 0);
+  return 0;
+}
+
 int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max) {
   if (!h) return fail("null handle");
   const auto& t = h->engine.trace;

@@ -55,6 +55,7 @@ Engine::Engine() { kind_stats.resize(8); }
 Engine::~Engine() {
   try {
     free_device();
+    destroy_graphs();
     for (auto& e : ev_pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   } catch (...) {
@@ -63,6 +64,7 @@ Engine::~Engine() {
 
 void Engine::free_device() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  destroy_graphs();
   dfree(d_sn_);
   dfree(d_lindx0_);
   dfree(d_tcol_);
@@ -79,6 +81,7 @@ void Engine::free_device() {
   dfree(d_scale_);
   dfree(d_arena64_);
   dfree(d_arena32_);
+  dfree(d_dinv_);
   dfree(d_Y_);
   dfree(d_X_);
   dfree(d_b_);
@@ -91,6 +94,25 @@ void Engine::free_device() {
 void Engine::init(const EngineOptions& o) {
   // no device work here: the host analysis runs without a GPU
   opt_ = o;
+}
+
+int64_t Engine::device_bytes(int nrhs) const {
+  auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
+  const Plan& p = plan_;
+  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.amap) + vb(p.perm0) +
+              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) +
+              vb(p.sdiag_tasks) + vb(p.soff_tasks);
+  b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
+  const int64_t elem = opt_.precision == 0 ? 8 : 4;
+  b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
+  b += (int64_t)n_ * std::max(1, nrhs) * (2 * elem + 3 * 8) + 16 * std::max(1, nrhs);
+  return b;
+}
+
+void Engine::release_device() {
+  free_device();
+  device_ready_ = have_values_ = factored_ = false;
+  cap_nrhs_ = 0;
 }
 
 void Engine::reset_stats() {
@@ -116,8 +138,8 @@ void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) 
 
 // Device set-up on first numeric use: stream, plan, maps, task lists, arena.
 void Engine::ensure_device() {
+  HIP_OK(hipSetDevice(opt_.device));  // the calling thread may not have selected it
   if (!stream_) {
-    HIP_OK(hipSetDevice(opt_.device));
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   }
   if (device_ready_) return;
@@ -148,6 +170,8 @@ void Engine::ensure_device() {
     d_arena64_ = dalloc<double>(plan_.arena_elems);
   else
     d_arena32_ = dalloc<float>(plan_.arena_elems);
+  // inverted 16x16 diagonal tiles (double-sized so both precisions fit)
+  d_dinv_ = dalloc<double>(std::max<int64_t>(1, plan_.inv_tiles) * kSub * kSub);
   HIP_OK(hipStreamSynchronize(s));
   device_ready_ = true;
 }
@@ -215,8 +239,21 @@ void Engine::assemble(T* arena) {
 }
 
 void Engine::launch_begin(int kind, int64_t grid) {
-  kind_stats[kind].launches++;
+  if (capturing_)
+    cap_kinds_.push_back(kind);
+  else
+    kind_stats[kind].launches++;
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
+  if (capturing_) {
+    // events owned by the graph, recorded by every replay
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    cap_events_.push_back({kind, {a, b}});
+    cap_grids_.push_back(grid);
+    HIP_OK(hipEventRecord(a, stream_));
+    return;
+  }
   ev_grid_.push_back(grid);
   while (ev_pool_.size() < ev_next_ + 2) {
     hipEvent_t e;
@@ -231,11 +268,114 @@ void Engine::launch_begin(int kind, int64_t grid) {
 
 void Engine::launch_end(int kind) {
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
+  if (capturing_) {
+    HIP_OK(hipEventRecord(cap_events_.back().second.second, stream_));
+    return;
+  }
   HIP_OK(hipEventRecord(ev_open_.back().second.second, stream_));
+}
+
+void Engine::destroy_graph(Graph& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  for (auto& e : g.events) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  g = Graph();
+}
+
+void Engine::destroy_graphs() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  pending_graphs_.clear();
+  destroy_graph(fgraph_);
+  for (auto& g : sgraphs_) destroy_graph(g);
+  sgraphs_.clear();
+}
+
+// Runs `body` (a sequence of launches on stream_) through the graph slot g:
+// captured on first use or when the key changes, replayed afterwards.
+template <typename F>
+void Engine::run_graph(Graph& g, int64_t key, F&& body) {
+  if (!use_graphs_) {
+    body();
+    return;
+  }
+  if (!g.exec || g.key != key) {
+    destroy_graph(g);
+    cap_kinds_.clear();
+    cap_events_.clear();
+    cap_grids_.clear();
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    capturing_ = true;
+    try {
+      body();
+    } catch (...) {
+      capturing_ = false;
+      (void)hipStreamEndCapture(stream_, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      for (auto& e : cap_events_) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+      }
+      cap_events_.clear();
+      (void)hipGetLastError();
+      if (!opt_.profile) throw;
+      // event records that cannot be captured: profile eagerly from now on
+      use_graphs_ = false;
+      destroy_graphs();
+      body();
+      return;
+    }
+    capturing_ = false;
+    HIP_OK(hipStreamEndCapture(stream_, &graph));
+    hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    g.events.swap(cap_events_);
+    g.grids.swap(cap_grids_);
+    g.kinds.swap(cap_kinds_);
+    if (e != hipSuccess) {
+      g.exec = nullptr;
+      destroy_graph(g);
+      HIP_OK(e);
+    }
+    g.key = key;
+  }
+  g.used = ++graph_clock_;
+  HIP_OK(hipGraphLaunch(g.exec, stream_));
+  for (int k : g.kinds) kind_stats[k].launches++;
+  if (!g.events.empty()) pending_graphs_.push_back(&g);
+}
+
+Engine::Graph& Engine::solve_graph(int64_t key) {
+  for (auto& g : sgraphs_)
+    if (g.key == key) return g;
+  constexpr size_t kMaxSolveGraphs = 4;
+  sgraphs_.reserve(kMaxSolveGraphs);  // pending_graphs_ keeps pointers into it
+  if (sgraphs_.size() < kMaxSolveGraphs) {
+    sgraphs_.emplace_back();
+    return sgraphs_.back();
+  }
+  Graph* lru = &sgraphs_[0];
+  for (auto& g : sgraphs_)
+    if (g.used < lru->used) lru = &g;
+  if (stream_) HIP_OK(hipStreamSynchronize(stream_));
+  pending_graphs_.clear();
+  destroy_graph(*lru);
+  return *lru;
 }
 
 void Engine::collect_events() {
   if (!opt_.profile) return;
+  for (Graph* g : pending_graphs_)
+    for (size_t i = 0; i < g->events.size(); ++i) {
+      auto& o = g->events[i];
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, o.second.first, o.second.second));
+      kind_stats[o.first].ms += ms;
+      if (trace.size() < 100000) trace.push_back({o.first, g->grids[i], ms});
+    }
+  pending_graphs_.clear();
   for (size_t i = 0; i < ev_open_.size(); ++i) {
     auto& o = ev_open_[i];
     float ms = 0.f;
@@ -248,6 +388,63 @@ void Engine::collect_events() {
   ev_next_ = 0;
 }
 
+template <typename T, bool LDL, int NBM>
+static void diag_launch(unsigned g, hipStream_t st, const SnInfo* sn, const Task4* t, T* arena, T* dinv,
+                        int* info) {
+  dev::k_diag<T, LDL, NBM><<<g, 256, 0, st>>>(sn, t, arena, dinv, info);
+}
+template <typename T, bool LDL, int NBM>
+static void ptrsm_launch(unsigned g, hipStream_t st, const SnInfo* sn, const Task4* t, T* arena,
+                         const T* dinv) {
+  dev::k_ptrsm<T, LDL, NBM><<<g, 256, 0, st>>>(sn, t, arena, dinv);
+}
+
+template <typename T>
+void Engine::launch_diag(int bucket, bool ldl, unsigned g, const Task4* t, T* arena) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  T* dinv = (T*)d_dinv_;
+#define SPK_DIAG(NBM)                                                 \
+  case NBM:                                                           \
+    if (ldl)                                                          \
+      diag_launch<T, true, NBM>(g, stream_, sn, t, arena, dinv, d_info_);  \
+    else                                                              \
+      diag_launch<T, false, NBM>(g, stream_, sn, t, arena, dinv, d_info_); \
+    break;
+  switch (bucket) {
+    SPK_DIAG(1)
+    SPK_DIAG(2)
+    SPK_DIAG(4)
+    SPK_DIAG(6)
+    SPK_DIAG(10)
+    default:
+      throw std::logic_error("bad diagonal-block bucket");
+  }
+#undef SPK_DIAG
+}
+
+template <typename T>
+void Engine::launch_ptrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const T* dinv = (const T*)d_dinv_;
+#define SPK_PTRSM(NBM)                                            \
+  case NBM:                                                       \
+    if (ldl)                                                      \
+      ptrsm_launch<T, true, NBM>(g, stream_, sn, t, arena, dinv);  \
+    else                                                          \
+      ptrsm_launch<T, false, NBM>(g, stream_, sn, t, arena, dinv); \
+    break;
+  switch (bucket) {
+    SPK_PTRSM(1)
+    SPK_PTRSM(2)
+    SPK_PTRSM(4)
+    SPK_PTRSM(6)
+    SPK_PTRSM(10)
+    default:
+      throw std::logic_error("bad diagonal-block bucket");
+  }
+#undef SPK_PTRSM
+}
+
 template <typename T>
 void Engine::factor_sequence(T* arena) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
@@ -258,18 +455,12 @@ void Engine::factor_sequence(T* arena) {
     switch (r.kind) {
       case 0: {
         const Task4* t = (const Task4*)d_tasks_[TA_POTRF] + r.first;
-        if (ldl)
-          dev::k_potrf<T, true><<<g, 64, 0, stream_>>>(sn, t, arena, d_info_);
-        else
-          dev::k_potrf<T, false><<<g, 64, 0, stream_>>>(sn, t, arena, d_info_);
+        launch_diag<T>(r.aux, ldl, g, t, arena);
         break;
       }
       case 7: {
         const Task4* t = (const Task4*)d_tasks_[TA_TRSM] + r.first;
-        if (ldl)
-          dev::k_trsm<T, true><<<g, 256, 0, stream_>>>(sn, t, arena);
-        else
-          dev::k_trsm<T, false><<<g, 256, 0, stream_>>>(sn, t, arena);
+        launch_ptrsm<T>(r.aux, ldl, g, t, arena);
         break;
       }
       case 1: {
@@ -298,15 +489,20 @@ void Engine::factor_sequence(T* arena) {
 
 int Engine::factorize() {
   if (!have_values_) throw std::logic_error("Factorize called before DistributeMatrix");
+  HIP_OK(hipSetDevice(opt_.device));
   auto t0 = std::chrono::steady_clock::now();
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
   if (opt_.precision == 0) {
-    assemble(d_arena64_);
-    factor_sequence(d_arena64_);
+    run_graph(fgraph_, 1, [&] {
+      assemble(d_arena64_);
+      factor_sequence(d_arena64_);
+    });
   } else {
-    assemble(d_arena32_);
-    factor_sequence(d_arena32_);
+    run_graph(fgraph_, 2, [&] {
+      assemble(d_arena32_);
+      factor_sequence(d_arena32_);
+    });
   }
   int info = 0;
   HIP_OK(hipMemcpyAsync(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -319,6 +515,10 @@ int Engine::factorize() {
 
 void Engine::ensure_solve_capacity(int nrhs) {
   if (nrhs <= cap_nrhs_) return;
+  // captured solves refer to the old buffers
+  for (auto& g : sgraphs_) destroy_graph(g);
+  sgraphs_.clear();
+  pending_graphs_.clear();
   dfree(d_Y_);
   dfree(d_X_);
   dfree(d_b_);
@@ -378,7 +578,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
 
 // r = b - A x (FP64, original numbering).
 void Engine::residual(const double* b, const double* x, double* r, int nrhs) {
-  HIP_OK(hipMemcpyAsync(r, b, (size_t)n_ * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  dev::k_copy<<<grid_for((int64_t)n_ * nrhs), 256, 0, stream_>>>(b, r, (int64_t)n_ * nrhs);
   const int cols_per_block = 4;
   unsigned g = (unsigned)((n_ + cols_per_block - 1) / cols_per_block);
   dev::k_residual<<<g, 64 * cols_per_block, 0, stream_>>>(d_acolptr_, d_arowind_, d_vals_, x, r, n_, nrhs);
@@ -411,9 +611,23 @@ __global__ void k_colmax2(const double* __restrict__ r, const double* __restrict
   }
 }
 
+// One refinement step: d = S (SAS)^{-1} S r in the factor's precision,
+// x += d, r = b - A x and the per-RHS norms, all on the device.
+template <typename T>
+void Engine::ir_step(T* Y, T* X, int nrhs) {
+  const unsigned g = grid_for((int64_t)n_ * nrhs);
+  dev::k_gather_rhs<T><<<g, 256, 0, stream_>>>(d_r_, Y, d_perm0_, n_, nrhs, d_scale_);
+  solve_sequence<T>(d_arena32_, Y, X, nrhs);
+  dev::k_scatter_sol<T><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
+  residual(d_b_, d_x_, d_r_, nrhs);
+  k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+  HIP_OK(hipGetLastError());
+}
+
 SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   if (!factored_) throw std::logic_error("Solve called without a successful Factorize");
   if (nrhs <= 0) throw std::invalid_argument("nrhs must be positive");
+  HIP_OK(hipSetDevice(opt_.device));
   auto t0 = std::chrono::steady_clock::now();
   ensure_solve_capacity(nrhs);
   const size_t cnt = (size_t)n_ * nrhs;
@@ -424,10 +638,12 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   if (opt_.precision == 0) {
     double* Y = (double*)d_Y_;
     double* X = (double*)d_X_;
-    dev::k_gather_rhs<double><<<g, 256, 0, stream_>>>(d_b_, Y, d_perm0_, n_, nrhs, nullptr);
-    solve_sequence<double>(d_arena64_, Y, X, nrhs);
-    dev::k_scatter_sol<double><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, nullptr, 0);
-    HIP_OK(hipGetLastError());
+    run_graph(solve_graph(4 * (int64_t)nrhs), 4 * (int64_t)nrhs, [&] {
+      dev::k_gather_rhs<double><<<g, 256, 0, stream_>>>(d_b_, Y, d_perm0_, n_, nrhs, nullptr);
+      solve_sequence<double>(d_arena64_, Y, X, nrhs);
+      dev::k_scatter_sol<double><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, nullptr, 0);
+      HIP_OK(hipGetLastError());
+    });
   } else {
     // FP64 iterative refinement around the FP32 factor of S A S
     // (S = diag(s)): x_0 = 0, r_0 = b; d = S (SAS)^{-1} S r; x += d;
@@ -444,11 +660,7 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
     for (int j = 0; j < nrhs; ++j) bnorm[j] = red[2 * j];
     rep.converged = 0;
     for (int it = 0; it < std::max(1, opt_.ir_max_iter); ++it) {
-      dev::k_gather_rhs<float><<<g, 256, 0, stream_>>>(d_r_, Y, d_perm0_, n_, nrhs, d_scale_);
-      solve_sequence<float>(d_arena32_, Y, X, nrhs);
-      dev::k_scatter_sol<float><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
-      residual(d_b_, d_x_, d_r_, nrhs);
-      k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+      run_graph(solve_graph(4 * (int64_t)nrhs + 1), 4 * (int64_t)nrhs + 1, [&] { ir_step<float>(Y, X, nrhs); });
       HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
       HIP_OK(hipStreamSynchronize(stream_));
       double berr = 0.0;

@@ -47,8 +47,15 @@ class Engine {
   void init(const EngineOptions& o);
   void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
   void set_profile(bool on, unsigned mask = 0xffu) {
+    if ((on ? 1 : 0) != opt_.profile || mask != profile_mask_) destroy_graphs();
     opt_.profile = on ? 1 : 0;
     profile_mask_ = mask;
+  }
+  // Replay the factor and solve launch sequences from captured HIP graphs
+  // (default) or launch every kernel from the host.
+  void set_graphs(bool on) {
+    if (!on) destroy_graphs();
+    use_graphs_ = on;
   }
   void symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval);
   void distribute(const double* nzval, bool on_device);
@@ -56,6 +63,12 @@ class Engine {
   SolveReport solve(const double* rhs, int nrhs, bool on_device);
   void get_solution(double* B, int nrhs, bool on_device);
 
+  // HBM bytes this engine holds after DistributeMatrix (plan, maps, values,
+  // factor) plus the solve buffers for nrhs right-hand sides.
+  int64_t device_bytes(int nrhs) const;
+  // Frees every device buffer; the symbolic analysis is kept, the values
+  // must be distributed again before the next factorization.
+  void release_device();
   const Symbolic& sym() const { return sym_; }
   const Plan& plan() const { return plan_; }
   const EngineOptions& options() const { return opt_; }
@@ -79,12 +92,42 @@ class Engine {
   template <typename T>
   void factor_sequence(T* arena);
   template <typename T>
+  void launch_diag(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
+  template <typename T>
+  void launch_ptrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
+  template <typename T>
   void solve_sequence(const T* arena, T* Y, T* X, int nrhs);
   template <typename T>
   void assemble(T* arena);
   void residual(const double* b, const double* x, double* r, int nrhs);
   void launch_begin(int kind, int64_t grid = 0);
   void launch_end(int kind);
+
+  // captured launch sequences
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    int64_t key = -1;
+    uint64_t used = 0;
+    std::vector<int> kinds;  // kernel kind of every captured launch (for the counters)
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // profile mode
+    std::vector<int64_t> grids;
+  };
+  template <typename F>
+  void run_graph(Graph& g, int64_t key, F&& body);
+  Graph& solve_graph(int64_t key);
+  void destroy_graph(Graph& g);
+  void destroy_graphs();
+  bool use_graphs_ = true;
+  bool capturing_ = false;
+  Graph fgraph_;
+  std::vector<Graph> sgraphs_;
+  uint64_t graph_clock_ = 0;
+  std::vector<Graph*> pending_graphs_;  // launched since the last collect_events
+  std::vector<int> cap_kinds_;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> cap_events_;
+  std::vector<int64_t> cap_grids_;
+  template <typename T>
+  void ir_step(T* Y, T* X, int nrhs);
 
   EngineOptions opt_;
   Symbolic sym_;
@@ -113,6 +156,7 @@ class Engine {
   double* d_vals_ = nullptr;
   double* d_scale_ = nullptr;  // equilibration s_i (FP32 mode)
   double* d_arena64_ = nullptr;
+  void* d_dinv_ = nullptr;     // inverted 16x16 diagonal tiles of the factor
   float* d_arena32_ = nullptr;
   // solve work
   int cap_nrhs_ = 0;

@@ -9,11 +9,11 @@
 // (include/sympack/symPACKMatrix2D.hpp:2096-2139).
 //
 // Kernels (one workgroup = 256 threads = 4 wave64):
-//  k_potrf       diagonal-block Cholesky/LDL^T in LDS (blockCell_t::factorize,
-//                symPACKMatrix2D.hpp:1052-1102).
-//  k_trsm        panel TRSM of one 64-row tile below that block
-//                (blockCell_t::trsm, :1104-1178).
-//  k_update<0>   trailing update of a wide panel inside itself (k = 64).
+//  k_diag        Cholesky/LDL^T of a diagonal block of up to 160 columns in
+//                LDS (blockCell_t::factorize, symPACKMatrix2D.hpp:1052-1102).
+//  k_ptrsm       panel TRSM of 64 rows below that block on MFMA with the
+//                inverted 16x16 diagonal tiles (blockCell_t::trsm, :1104-1178).
+//  k_update<0>   trailing update of a panel wider than 160 inside itself.
 //  k_update<1>   Schur update of ancestors: -P P^T (LDL: -P D P^T) on FP64 /
 //                FP32 MFMA 16x16x4, scattered straight from registers into the
 //                target panels through precomputed relative indices with
@@ -86,6 +86,13 @@ __global__ void k_fill_zero(T* __restrict__ p, int64_t n) {
   for (; i < n; i += stride) p[i] = T(0);
 }
 
+template <typename T>
+__global__ void k_copy(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // arena[amap[e]] = scale * vals[e]; amap[e] < 0 marks a superseded duplicate.
 // With equilibration, scale = s_row * s_col per entry (esc != nullptr).
 template <typename T>
@@ -103,142 +110,331 @@ __global__ void k_assemble(T* __restrict__ arena, const int64_t* __restrict__ am
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block factorization (bs <= 64), one wave per block.  Lane i keeps
-// row i of the block in registers; at step j the unscaled pivot column is
-// broadcast through LDS and every lane applies the rank-1 update
-// a_ik -= a_ij a_kj / a_jj to its own row (the same formula serves LL^T and
-// LDL^T); the column is scaled at the end of the step.  Task {sn, step, -, -};
-// info receives the smallest failing pivot column (1-based, permuted order).
+// Scalar helpers for the pivots: hardware reciprocal / reciprocal square root
+// refined by Newton-Raphson (y <- y(3 - a y^2)/2, y <- y(2 - a y)), two steps
+// for double, one for float.
 // ---------------------------------------------------------------------------
-template <typename T, bool LDL>
-__global__ __launch_bounds__(64) void k_potrf(const SnInfo* __restrict__ sn,
-                                               const Task4* __restrict__ tasks,
-                                               T* __restrict__ arena, int* __restrict__ info) {
-  __shared__ __attribute__((aligned(16))) T col[NB];
-  const Task4 t = tasks[blockIdx.x];
-  const SnInfo s = sn[t.a];
-  const int lane = threadIdx.x;
-  const int c0 = t.b * NB;
-  const int bs = min(NB, s.w - c0);
-  T* __restrict__ P = arena + s.off;
-  const int64_t ld = s.r;
-  // clamped, unconditional loads keep all 64 loads in flight; entries
-  // outside the block's lower triangle are zero, so the update loops below
-  // need no per-element guards.
-  T a[NB];
-  const int lr = min(lane, bs - 1);
-#pragma unroll
-  for (int k = 0; k < NB; ++k) a[k] = P[(int64_t)(c0 + min(k, bs - 1)) * ld + c0 + lr];
-#pragma unroll
-  for (int k = 0; k < NB; ++k) keep(a[k]);
-#pragma unroll
-  for (int k = 0; k < NB; ++k) a[k] = (lane < bs && k <= lane) ? a[k] : T(0);
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < bs) {
-      col[lane] = a[j];
-      __syncthreads();
-      T ajj = col[j];
-      const bool bad = LDL ? (ajj == T(0) || ajj != ajj) : !(ajj > T(0));
-      if (bad) {
-        if (lane == 0) atomicMin(info, s.fc + c0 + j + 1);
-        ajj = T(1);
-      }
-      const T inv = T(1) / ajj;
-      const T t2 = a[j] * inv;
-#pragma unroll
-      for (int k = j + 1; k < NB; ++k) a[k] -= t2 * col[k];
-      if (LDL)
-        a[j] = (lane == j) ? ajj : t2;
-      else
-        a[j] = (lane == j) ? sqrt(ajj) : a[j] * (T(1) / sqrt(ajj));
-      __syncthreads();
-    }
-  }
-  if (lane < bs) {
-    T* prow = P + (int64_t)c0 * ld + c0 + lane;
-    asm volatile("" : "+v"(prow));
-#pragma unroll
-    for (int k = 0; k < NB; ++k)
-      if (k <= lane) prow[(int64_t)k * ld] = a[k];
-  }
+__device__ __forceinline__ double rsqrt_nr(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  const double h = 0.5 * a;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float a) {
+  float y = __builtin_amdgcn_rsqf(a);
+  y = y * (1.5f - 0.5f * a * y * y);
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double a) {
+  double y = __builtin_amdgcn_rcp(a);
+  y = y * (2.0 - a * y);
+  y = y * (2.0 - a * y);
+  return y;
+}
+__device__ __forceinline__ float rcp_nr(float a) {
+  float y = __builtin_amdgcn_rcpf(a);
+  y = y * (2.0f - a * y);
+  return y;
+}
+
+__device__ __forceinline__ double readlane_t(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float readlane_t(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // ---------------------------------------------------------------------------
-// Panel TRSM of a 256-row tile below a factored diagonal block:
-// B <- B L^{-T} (LL) or B <- B L^{-T} D^{-1} (LDL, unit L).  The block is
-// staged in LDS by columns; each thread then solves its own row from
-// registers with broadcast LDS reads, without further barriers.
-// Task {sn, step, row_tile, -}.
+// Diagonal blocks are held in LDS as the lower triangle of 16x16 tiles:
+// tile (I,J), I >= J, at slot I(I+1)/2 + J, column-major with column stride
+// 17 (conflict-free both along columns and along rows).
 // ---------------------------------------------------------------------------
-constexpr int TRSM_ROWS = 256;
+constexpr int SUB = kSub;
+constexpr int TSTR = SUB + 1;
+constexpr int TSZ = SUB * TSTR;
+__host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 + J; }
 
-// Body of the TRSM for a diagonal block of width bs <= W (W = 16, 32, 64):
-// the unrolled loops only run over W columns.
-template <typename T, bool LDL, int W>
-__device__ __forceinline__ void trsm_rows(T* __restrict__ P, int64_t ld, int c0, int bs, int R0,
-                                          int nrows_panel, T (*Lc)[NB], const T* invd) {
-  const int tid = threadIdx.x;
-  // rows past the panel work on a clamped (valid) row and do not store
-  const bool active = R0 + tid < nrows_panel;
-  const int row = min(R0 + tid, nrows_panel - 1);
-  // columns >= bs read column bs-1 (a valid address) and are zeroed; Lc is
-  // zero on and above its diagonal and invd is zero beyond bs, so the
-  // substitution needs no per-element guards.
-  T x[W];
-  {
-    const T* prow = P + (int64_t)c0 * ld + row;
-#pragma unroll
-    for (int j = 0; j < W; ++j) x[j] = prow[(int64_t)min(j, bs - 1) * ld];
-  }
-#pragma unroll
-  for (int j = 0; j < W; ++j) keep(x[j]);
-#pragma unroll
-  for (int j = 0; j < W; ++j) x[j] = (j < bs) ? x[j] : T(0);
-#pragma unroll
-  for (int j = 0; j < W; ++j) {
-    const T xj = LDL ? x[j] : x[j] * invd[j];
-#pragma unroll
-    for (int k = j + 1; k < W; ++k) x[k] -= xj * Lc[j][k];
-  }
-  if (!active) return;
-  // a fresh row pointer: the load addresses must not stay live until here
-  T* prow = P + (int64_t)c0 * ld + row;
-  asm volatile("" : "+v"(prow));
-  // unconditional stores in descending column order: columns >= bs write
-  // zeros to column bs-1, which its own (correct) value then overwrites; a
-  // guard per column would split the code into many blocks and spill
-#pragma unroll
-  for (int j = W - 1; j >= 0; --j) prow[(int64_t)min(j, bs - 1) * ld] = x[j] * invd[j];
-}
-
-template <typename T, bool LDL>
-__global__ __launch_bounds__(NT) void k_trsm(const SnInfo* __restrict__ sn,
+// ---------------------------------------------------------------------------
+// Factorization of one diagonal block (bs <= 160 columns of a panel step),
+// one workgroup per block, the block resident in LDS
+// (blockCell_t::factorize, symPACKMatrix2D.hpp:1052-1102; LDL^T:
+// blockCellLDL_t::factorize, :2096-2116, lapack.hpp:206-617).
+// Right-looking over 16-column block columns J:
+//  1. every wave factors block column J lane-per-row: lanes 0-15 hold the
+//     16 rows of the diagonal tile (each wave keeps its own copy, so pivots
+//     are broadcast with readlane and need no barrier), lanes 16-63 hold 48
+//     rows below it; wave 1 also forms the inverse of the diagonal tile,
+//     which the panel TRSM uses;
+//  2. the trailing lower tiles get A_IK -= L_IJ L_KJ^T (LDL: L_IJ D_J L_KJ^T)
+//     on MFMA 16x16x4, tiles spread over the 4 waves.
+// Rows/columns beyond bs are padded with an identity block.  Task
+// {sn, step, inverse-tile offset, -}; info receives the smallest failing
+// pivot column (1-based, permuted order).
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL, int NBM>
+__global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
                                               const Task4* __restrict__ tasks,
-                                              T* __restrict__ arena) {
-  __shared__ __attribute__((aligned(16))) T Lc[NB][NB];  // Lc[j][k] = L(k, j), k > j
-  __shared__ T invd[NB];
+                                              T* __restrict__ arena, T* __restrict__ dinv,
+                                              int* __restrict__ info) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NTL = NBM * (NBM + 1) / 2;
+  __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
+  __shared__ T dsh[SUB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
-  const int tid = threadIdx.x;
-  const int c0 = t.b * NB;
-  const int bs = min(NB, s.w - c0);
-  T* __restrict__ P = arena + s.off;
+  const int c0 = t.b * kStep;
+  const int bs = min(kStep, s.w - c0);
+  const int nb = (bs + SUB - 1) / SUB;
   const int64_t ld = s.r;
-  const int R0 = c0 + bs + t.c * TRSM_ROWS;
-  for (int idx = tid; idx < NB * NB; idx += NT) {
-    const int k = idx % NB, j = idx / NB;  // column j, row k: coalesced down the column
-    const T v = P[(int64_t)(c0 + min(j, bs - 1)) * ld + c0 + min(k, bs - 1)];
-    Lc[j][k] = (k < bs && j < bs && k > j) ? v : T(0);
-    if (k == j) invd[j] = (j < bs) ? T(1) / v : T(0);
+  T* __restrict__ P = arena + s.off + (int64_t)c0 * ld + c0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // ---- load the lower tiles (clamped unconditional loads, one wait) ----
+  {
+    const int r = tid & 15, c = tid >> 4;
+    T v[NTL];
+#pragma unroll
+    for (int I = 0; I < NBM; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) {
+        const int row = min(SUB * I + r, bs - 1), col = min(SUB * J + c, bs - 1);
+        v[tslot(I, J)] = P[(int64_t)col * ld + row];
+      }
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) keep(v[q]);
+#pragma unroll
+    for (int I = 0; I < NBM; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) {
+        const int row = SUB * I + r, col = SUB * J + c;
+        const bool in = row < bs && col < bs && row >= col;
+        tl[tslot(I, J) * TSZ + c * TSTR + r] = in ? v[tslot(I, J)] : T(row == col ? 1 : 0);
+      }
   }
   __syncthreads();
-  if (bs <= 16)
-    trsm_rows<T, LDL, 16>(P, ld, c0, bs, R0, s.r, Lc, invd);
-  else if (bs <= 32)
-    trsm_rows<T, LDL, 32>(P, ld, c0, bs, R0, s.r, Lc, invd);
-  else
-    trsm_rows<T, LDL, 64>(P, ld, c0, bs, R0, s.r, Lc, invd);
+
+  const bool diag_lane = lane < SUB;
+  const int rb = 48 * wave + lane - SUB;  // row below the diagonal tile (lanes >= 16)
+  for (int J = 0; J < nb; ++J) {
+    // ---- 1. block column J, lane per row ----
+    const int Ib = J + 1 + (diag_lane ? 0 : rb / SUB);
+    const bool row_ok = diag_lane || (Ib < nb);
+    const int I = diag_lane ? J : min(Ib, nb - 1);
+    const int rr = diag_lane ? lane : rb % SUB;
+    T* trow = tl + tslot(I, J) * TSZ + rr;
+    T a[SUB];
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
+    T yv[SUB];  // 1/L_jj (LL) or 1/D_j (LDL)
+#pragma unroll
+    for (int j = 0; j < SUB; ++j) {
+      T piv = readlane_t(a[j], j);
+      const bool bad = LDL ? (piv == T(0) || piv != piv) : !(piv > T(0));
+      if (bad) {
+        if (SUB * J + j < bs && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + SUB * J + j + 1);
+        piv = T(1);
+      }
+      if (LDL) {
+        const T inv = rcp_nr(piv);
+        yv[j] = inv;
+        const T l = a[j] * inv;
+#pragma unroll
+        for (int k = j + 1; k < SUB; ++k) a[k] -= l * readlane_t(a[j], k);
+        a[j] = (lane == j) ? piv : l;
+      } else {
+        const T y = rsqrt_nr(piv);
+        yv[j] = y;
+        const T t2 = a[j] * y * y;
+#pragma unroll
+        for (int k = j + 1; k < SUB; ++k) a[k] -= t2 * readlane_t(a[j], k);
+        a[j] = a[j] * y;
+      }
+    }
+    // diagonal tile: lower part only (wave 0); rows below: every valid lane
+    if ((diag_lane && wave == 0) || (!diag_lane && row_ok)) {
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (diag_lane && c > lane) ? T(0) : a[c];
+    }
+    if (LDL && wave == 0 && diag_lane) dsh[lane] = a[lane];
+    if (wave == 1) {
+      // inverse of the diagonal tile, column `lane` per lane (forward
+      // substitution L z = e_lane); LDL: unit lower triangle
+      T z[SUB];
+#pragma unroll
+      for (int i = 0; i < SUB; ++i) {
+        T acc = (lane == i) ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc -= readlane_t(a[k], i) * z[k];
+        z[i] = LDL ? acc : acc * yv[i];
+      }
+      if (diag_lane) {
+        T* out = dinv + (int64_t)(t.c + J) * (SUB * SUB) + lane * SUB;
+#pragma unroll
+        for (int i = 0; i < SUB; ++i) out[i] = z[i];
+      }
+    }
+    __syncthreads();
+    // ---- 2. trailing update of tiles (I,K), J < K <= I < nb ----
+    const int m = nb - J - 1;
+    const int ntr = m * (m + 1) / 2;
+    for (int q = wave; q < ntr; q += NT / 64) {
+      // decode q -> (K, I): columns K = J+1.., rows I = K..nb-1
+      int K = J + 1, rem = q, len = m;
+      while (rem >= len) {
+        rem -= len;
+        --len;
+        ++K;
+      }
+      const int It = K + rem;
+      T* C = tl + tslot(It, K) * TSZ;
+      const T* A = tl + tslot(It, J) * TSZ;
+      const T* B = tl + tslot(K, J) * TSZ;
+      v4 acc;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        T av = A[kp * TSTR + (lane & 15)];
+        if (LDL) av *= dsh[kp];
+        const T bv = B[kp * TSTR + (lane & 15)];
+        acc = Mfma<T>::mma(-av, bv, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[v];
+    }
+    __syncthreads();
+  }
+
+  // ---- write the factored lower triangle back ----
+  {
+    const int r = tid & 15, c = tid >> 4;
+#pragma unroll
+    for (int I = 0; I < NBM; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) {
+        const int row = SUB * I + r, col = SUB * J + c;
+        if (row < bs && col < bs && row >= col)
+          P[(int64_t)col * ld + row] = tl[tslot(I, J) * TSZ + c * TSTR + r];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Panel TRSM of 64 rows below a factored diagonal block (blockCell_t::trsm,
+// symPACKMatrix2D.hpp:1104-1178; LDL: blockCellLDL_t::trsm, :2118-2139):
+// X = B L^{-T} (LDL: B L^{-T} D^{-1}, unit L), blocked by 16 columns with the
+// inverted diagonal tiles of k_diag:
+//   Y_K = inv(L_KK) (B_K^T - sum_{J<K} L_KJ Y_J),  X_K^T = Y_K (LDL: D_K^{-1} Y_K).
+// Each wave owns 16 rows; Y^T lives in MFMA accumulators, and the f64/f32
+// accumulator layout doubles as the B operand of the next products (slot kk
+// of tile J is k-chunk kk), so no data moves between the steps.
+// Task {sn, step, row_tile, inverse-tile offset}.
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL, int NBM>
+__global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
+                                               const Task4* __restrict__ tasks,
+                                               T* __restrict__ arena, const T* __restrict__ dinv) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NTL = NBM * (NBM + 1) / 2;
+  __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];  // (I,J) I>J: L; (I,I): inv(L_II)
+  __shared__ T dd[NBM * SUB];                               // LDL: 1/D
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int c0 = t.b * kStep;
+  const int bs = min(kStep, s.w - c0);
+  const int nb = (bs + SUB - 1) / SUB;
+  const int64_t ld = s.r;
+  T* __restrict__ P = arena + s.off + (int64_t)c0 * ld;  // column c0 of the panel
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {
+    const int r = tid & 15, c = tid >> 4;
+    const T* Dv = dinv + (int64_t)t.d * (SUB * SUB);
+    T v[NTL];
+#pragma unroll
+    for (int I = 0; I < NBM; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) {
+        if (I == J) {
+          v[tslot(I, J)] = Dv[(int64_t)min(I, nb - 1) * (SUB * SUB) + c * SUB + r];
+        } else {
+          const int row = min(SUB * I + r, bs - 1), col = min(SUB * J + c, bs - 1);
+          v[tslot(I, J)] = P[(int64_t)col * ld + c0 + row];
+        }
+      }
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) keep(v[q]);
+#pragma unroll
+    for (int I = 0; I < NBM; ++I)
+#pragma unroll
+      for (int J = 0; J <= I; ++J) {
+        const int row = SUB * I + r, col = SUB * J + c;
+        const bool in = (I == J) ? (I < nb) : (row < bs && col < bs);
+        tl[tslot(I, J) * TSZ + c * TSTR + r] = in ? v[tslot(I, J)] : T(0);
+      }
+    if (LDL && tid < NBM * SUB) {
+      const int j = min(tid, bs - 1);
+      const T d = P[(int64_t)j * ld + c0 + j];
+      dd[tid] = (tid < bs) ? T(1) / d : T(0);
+    }
+  }
+  __syncthreads();
+
+  // rows of this wave: R0 + (lane & 15), columns as MFMA rows
+  const int R0 = c0 + bs + t.c * kTrsmRows + 16 * wave;
+  if (R0 >= s.r) return;  // no barrier follows
+  const int prow = min(R0 + (lane & 15), s.r - 1);
+  const bool prow_ok = R0 + (lane & 15) < s.r;
+  v4 y[NBM];
+#pragma unroll
+  for (int K = 0; K < NBM; ++K)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int col = min(SUB * K + Mfma<T>::row(lane, v), bs - 1);
+      y[K][v] = P[(int64_t)col * ld + prow];
+    }
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) asm volatile("" : "+v"(y[K]));
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) {
+    if (K < nb) {
+#pragma unroll
+      for (int J = 0; J < K; ++J) {
+        const T* L = tl + tslot(K, J) * TSZ + (lane & 15);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const int kp = Mfma<T>::row(lane, kk);
+          y[K] = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y[K]);
+        }
+      }
+      const T* Li = tl + tslot(K, K) * TSZ + (lane & 15);
+      v4 z = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        z = Mfma<T>::mma(Li[kp * TSTR], y[K][kk], z);
+      }
+      y[K] = z;
+    }
+  }
+  if (!prow_ok) return;
+  T* __restrict__ out = P + R0 + (lane & 15);
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) {
+    if (K < nb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = SUB * K + Mfma<T>::row(lane, v);
+        T x = y[K][v];
+        if (LDL) x *= dd[min(col, NBM * SUB - 1)];
+        if (col < bs) out[(int64_t)col * ld] = x;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -270,8 +466,8 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   const int64_t ld = s.r;
   int kbeg, K, arow0, brow0, alim, blim, ti, tj;
   if (MODE == 0) {
-    kbeg = t.b * NB;
-    K = min(NB, s.w - kbeg);
+    kbeg = t.b * kStep;
+    K = min(kStep, s.w - kbeg);
     const int rb = kbeg + K;
     ti = t.c;
     tj = t.d;
@@ -443,15 +639,6 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 // ---------------------------------------------------------------------------
 constexpr int JC = 64;  // right-hand sides per LDS chunk
 
-__device__ __forceinline__ double readlane_t(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float readlane_t(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
 
 // Triangular solve of one diagonal block (bs <= 64) for right-hand sides
 // handled one per wave: lane i holds x_i, the pivot value is broadcast with

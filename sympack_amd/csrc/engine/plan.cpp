@@ -128,22 +128,27 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   for (int L = 0; L < p.nlevels; ++L) {
     const auto& sns = by_level[L];
     int smax = 0;
-    for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kNB));
+    for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kStep));
     for (int st = 0; st < smax; ++st) {
       LaunchRange pr{0, (int64_t)p.potrf_tasks.size(), 0};
       LaunchRange tr{7, (int64_t)p.trsm_tasks.size(), 0};
       LaunchRange ur{1, (int64_t)p.inpanel_tasks.size(), 0};
+      int nbmax = 1;
       for (Int I : sns) {
         const SnInfo& si = p.sn[I];
-        int nb = (int)ceil_div(si.w, kNB);
-        if (st >= nb) continue;
-        int c0 = st * kNB, bs = std::min(kNB, si.w - c0);
+        int nsteps = (int)ceil_div(si.w, kStep);
+        if (st >= nsteps) continue;
+        int c0 = st * kStep, bs = std::min(kStep, si.w - c0);
+        int nb = (int)ceil_div(bs, kSub);
+        nbmax = std::max(nbmax, nb);
         int64_t rows_below = si.r - c0 - bs;
-        p.potrf_tasks.push_back({I, st, 0, 0});
+        int32_t inv_off = (int32_t)p.inv_tiles;
+        p.inv_tiles += nb;
+        p.potrf_tasks.push_back({I, st, inv_off, 0});
         for (int64_t rt = 0; rt < ceil_div(rows_below, kTrsmRows); ++rt)
-          p.trsm_tasks.push_back({I, st, (int32_t)rt, 0});
+          p.trsm_tasks.push_back({I, st, (int32_t)rt, inv_off});
         p.diag_flops += (double)bs * bs * bs / 3.0 + (double)bs * bs * rows_below;
-        if (st < nb - 1) {
+        if (st < nsteps - 1) {
           int rb = c0 + bs;  // trailing rows/cols start
           int64_t nrow = si.r - rb, ncol = si.w - rb;
           int64_t ntr = ceil_div(nrow, kTile), ntc = ceil_div(ncol, kTile);
@@ -154,6 +159,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
           p.inpanel_flops += (double)bs * ((double)ncol * (ncol + 1) + 2.0 * (nrow - ncol) * ncol);
         }
       }
+      pr.aux = tr.aux = diag_bucket(nbmax);
       pr.count = (int64_t)p.potrf_tasks.size() - pr.first;
       tr.count = (int64_t)p.trsm_tasks.size() - tr.first;
       ur.count = (int64_t)p.inpanel_tasks.size() - ur.first;

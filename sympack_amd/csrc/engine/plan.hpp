@@ -19,9 +19,15 @@
 
 namespace spk {
 
-constexpr int kNB = 64;    // diagonal block width of the blocked panel factorization
-constexpr int kTile = 64;  // update tile edge (rows x cols)
-constexpr int kTrsmRows = 256;  // panel rows per TRSM workgroup
+constexpr int kNB = 64;      // diagonal block width of the blocked solve
+constexpr int kTile = 64;    // update tile edge (rows x cols)
+constexpr int kStep = 160;   // panel columns factored per step (diagonal block <= 160 x 160)
+constexpr int kSub = 16;     // sub-block edge inside a diagonal block (MFMA 16x16 tiles)
+constexpr int kTrsmRows = 64;  // panel rows per TRSM workgroup (16 per wave)
+
+// LDS bucket of a diagonal block with nb 16-column sub-blocks: kernels are
+// compiled for nb <= 1, 2, 4, 6, 10.
+inline int diag_bucket(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 6 ? 6 : 10; }
 
 struct SnInfo {            // mirrored on the device (16-byte aligned)
   int64_t off;             // panel offset in the arena (elements)
@@ -41,6 +47,7 @@ struct LaunchRange {
                      // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
+  int aux = 0;       // kinds 0 and 7: diagonal-block bucket (diag_bucket)
 };
 
 struct Plan {
@@ -59,8 +66,9 @@ struct Plan {
   std::vector<int64_t> amap;
   std::vector<int32_t> perm0;      // perm0[new] = old, 0-based
   // task arrays
-  std::vector<Task4> potrf_tasks;  // {sn, step, 0, 0}
-  std::vector<Task4> trsm_tasks;   // {sn, step, row_tile, 0}
+  std::vector<Task4> potrf_tasks;  // {sn, step, inverse-tile offset, 0}
+  std::vector<Task4> trsm_tasks;   // {sn, step, row_tile, inverse-tile offset}
+  int64_t inv_tiles = 0;           // 16x16 inverse diagonal tiles written by the factor
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
   std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}
   std::vector<LaunchRange> factor_seq;

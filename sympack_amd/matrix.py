@@ -267,6 +267,20 @@ class symPACKMatrix2D:
                 mask |= 1 << k
             _lib.check(self._lib.spk_set_profile_mask(self._h, 1 if on else 0, mask), "set_profile")
 
+    def DeviceBytes(self, nrhs=1):
+        """HBM bytes held after DistributeMatrix plus solve buffers for nrhs."""
+        b = ctypes.c_int64()
+        _lib.check(self._lib.spk_device_bytes(self._h, int(nrhs), ctypes.byref(b)), "DeviceBytes")
+        return b.value
+
+    def ReleaseDevice(self):
+        """Free the device memory; DistributeMatrix must be called again."""
+        _lib.check(self._lib.spk_release_device(self._h), "ReleaseDevice")
+
+    def set_graphs(self, on):
+        """Replay launch sequences from captured HIP graphs (default) or not."""
+        _lib.check(self._lib.spk_set_graphs(self._h, 1 if on else 0), "set_graphs")
+
     def TotalSupernodeCnt(self):
         return self.stats()["nsuper"]
 

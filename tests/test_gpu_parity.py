@@ -73,7 +73,7 @@ def test_factor_panels_and_solve_ll(name, gen, ordering, bmax):
     assert be_g <= 10 * max(be_o, 1e-16)
 
 
-@pytest.mark.parametrize("name,gen,ordering,bmax", CASES[:4], ids=[c[0] for c in CASES[:4]])
+@pytest.mark.parametrize("name,gen,ordering,bmax", CASES[:5], ids=[c[0] for c in CASES[:5]])
 def test_ldl_matches_oracle(name, gen, ordering, bmax):
     A, Mx, S, F, X, Xo, B, xt = _run(gen, ordering, ldl=True, relax_max=bmax, nrhs=3)
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
@@ -105,8 +105,10 @@ def test_not_spd_raises():
     Mx.Init(sp.symPACKOptions(ordering="MMD"))
     Mx.SymbolicFactorization(A)
     Mx.DistributeMatrix(A)
-    with pytest.raises(sp.NotHPDError):
+    with pytest.raises(sp.NotHPDError) as e:
         Mx.Factorize()
+    info, F = onum.factor(onum.Structure.of(Mx), A)
+    assert e.value.column == info > 0
 
 
 def test_fp32_ir_reaches_fp64_accuracy():
@@ -126,3 +128,97 @@ def test_fp32_ir_reaches_fp64_accuracy():
     st = Mx.stats()
     assert st["backward_error"] <= 1e-12
     assert onum.backward_error(A, X, B) <= 1e-12
+
+
+def test_fp32_ir_ldl_indefinite():
+    n, cp, ri, v = M.shifted_laplacian_3d_7pt(12, 0.3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", relax=rp, precision="FP32_IR",
+                              ir_tol=1e-12))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    rng = np.random.default_rng(6)
+    B = rng.uniform(-1, 1, (n, 3))
+    Mx.Solve(B, 3)
+    X = Mx.GetSolution(3)
+    assert Mx.stats()["backward_error"] <= 1e-12
+    assert onum.backward_error(A, X, B) <= 1e-12
+
+
+def _panels(Mx, S):
+    return [Mx.GetFactorPanel(I) for I in range(1, S.nsuper + 1)]
+
+
+def _rel(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(y ** 2)) for y in b)
+    return np.sqrt(num / den)
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32_IR"])
+def test_graph_replays_match_eager(prec):
+    n, cp, ri, v = M.diffusion_3d_7pt(12, seed=21)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+
+    def make(graphs):
+        Mx = sp.symPACKMatrix2D()
+        Mx.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision=prec))
+        Mx.set_graphs(graphs)
+        Mx.SymbolicFactorization(A)
+        Mx.DistributeMatrix(A)
+        return Mx
+
+    rng = np.random.default_rng(3)
+    B = rng.uniform(-1, 1, (n, 2))
+    E = make(False)
+    E.Factorize()
+    S = onum.Structure.of(E)
+    P0 = _panels(E, S)
+    E.Solve(B, 2)
+    X0 = E.GetSolution(2)
+    G = make(True)
+    tol = 1e-12 if prec == "FP64" else 1e-5
+    for rep in range(3):
+        print("graph replay", prec, rep, flush=True)
+        G.Factorize()
+        assert _rel(_panels(G, S), P0) < tol
+        G.Solve(B, 2)
+        X = G.GetSolution(2)
+        assert np.linalg.norm(X - X0) / np.linalg.norm(X0) < 1e-10
+        # eager work and a synchronization between replays
+        E.Solve(B[:, :1], 1)
+        E.GetSolution(1)
+    # the launch counters see every replayed kernel
+    G.reset_kernel_stats()
+    E.reset_kernel_stats()
+    G.Factorize()
+    E.Factorize()
+    for k in range(8):
+        assert G.kernel_stats(k)[0] == E.kernel_stats(k)[0]
+
+
+def test_profiled_graph_records_kernel_times():
+    n, cp, ri, v = M.diffusion_3d_7pt(10, seed=22)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    Mx.reset_kernel_stats()
+    Mx.set_profile(True, kinds=[2])
+    for _ in range(3):
+        Mx.Factorize()
+    Mx.set_profile(False)
+    launches, ms = Mx.kernel_stats(2)
+    assert launches > 0 and ms > 0
+    kinds, grids, times = Mx.launch_trace()
+    assert len(kinds) == launches and set(kinds) == {2} and np.all(times > 0)
