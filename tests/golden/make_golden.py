@@ -1,5 +1,5 @@
 """Writes tests/golden/orderings.npz: ordering vectors produced by symPACK's
-own Fortran routines (ordmmd/genmmd, genrcm, boxnd, gridnd), compiled from the
+own Fortran routines (ordmmd/genmmd, amdbar, genrcm, boxnd, gridnd), compiled from the
 reference sources by oracle/Makefile into oracle/_ref/libref_orderings.so.
 
 Run from the repository root (needs the reference tree and the compiled
@@ -58,6 +58,24 @@ def ref_mmd(lib, n, cp, ri):
     return invp  # Ordering::MMD keeps invp and rebuilds perm from it
 
 
+def ref_amd(lib, n, cp, ri):
+    """amdbar_ called with the arrays Ordering::AMD builds (Ordering.cpp:
+    456-486): IWLEN = nnz + n, PFREE = nnz + 1; ELEN returns invp."""
+    xadj, adjncy = graph_nodiag(n, cp, ri)
+    nnz = len(adjncy)
+    pe = xadj.copy()
+    iw = np.zeros(nnz + n, np.int32)
+    iw[:nnz] = adjncy
+    ln = np.diff(xadj).astype(np.int32)
+    nv, nxt, last, head, elen, deg, w = (np.zeros(n, np.int32) for _ in range(7))
+    I = ctypes.c_int32
+    ncmpa = I(0)
+    lib.amdbar_(ctypes.byref(I(n)), p32(pe), p32(iw), p32(ln), ctypes.byref(I(nnz + n)),
+                ctypes.byref(I(nnz + 1)), p32(nv), p32(nxt), p32(last), p32(head), p32(elen), p32(deg),
+                ctypes.byref(ncmpa), p32(w), ctypes.byref(I(2 ** 31 - 1)))
+    return elen
+
+
 def ref_rcm(lib, n, cp, ri):
     xadj, adjncy = graph_nodiag(n, cp, ri)
     perm = np.zeros(2 * n, np.int32)
@@ -104,6 +122,7 @@ def main():
         out[name + "_colptr"] = cp.astype(np.int64)
         out[name + "_rowind"] = ri.astype(np.int32)
         out[name + "_mmd_invp"] = ref_mmd(lib, n, cp, ri)
+        out[name + "_amd_invp"] = ref_amd(lib, n, cp, ri)
         out[name + "_rcm_perm"] = ref_rcm(lib, n, cp, ri)
     for k in (4, 6, 9, 12, 20):
         out["boxnd_%d" % k] = ref_boxnd(lib, k)
@@ -113,6 +132,11 @@ def main():
     for k in (48, 60):
         box = ref_boxnd(lib, k)
         out["boxnd_%d_sha256" % k] = np.frombuffer(hashlib.sha256(box.tobytes()).digest(), np.uint8)
+    # larger AMD cases (workspace compaction): checksum of invp only
+    for name, (n, cp, ri, v) in {"st27_20": M.stencil_3d_27pt(20, seed=4),
+                                 "diff3d_30": M.diffusion_3d_7pt(30, seed=5)}.items():
+        invp = ref_amd(lib, n, cp, ri)
+        out["amd_%s_sha256" % name] = np.frombuffer(hashlib.sha256(invp.tobytes()).digest(), np.uint8)
     path = os.path.join(ROOT, "tests", "golden", "orderings.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, len(out), "arrays")

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import sympack_amd as sp
+from sympack_amd import matrices as M
 from oracle import symbolic as O
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "orderings.npz"))
@@ -26,6 +27,22 @@ def test_mmd_matches_reference_fortran(name):
     perm, invp = sp.compute_ordering("MMD", _A(name))
     assert np.array_equal(invp, G[name + "_mmd_invp"])
     assert np.array_equal(np.argsort(invp) + 1, perm)
+
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_amd_matches_reference_fortran(name):
+    perm, invp = sp.compute_ordering("AMD", _A(name))
+    assert np.array_equal(invp, G[name + "_amd_invp"])
+    assert np.array_equal(np.argsort(invp) + 1, perm)
+
+
+@pytest.mark.parametrize("name,gen", [("st27_20", lambda: M.stencil_3d_27pt(20, seed=4)),
+                                      ("diff3d_30", lambda: M.diffusion_3d_7pt(30, seed=5))])
+def test_amd_large_checksum(name, gen):
+    n, cp, ri, v = gen()
+    perm, invp = sp.compute_ordering("AMD", sp.DistSparseMatrix(n, cp, ri, v))
+    digest = hashlib.sha256(invp.astype(np.int32).tobytes()).digest()
+    assert digest == G["amd_%s_sha256" % name].tobytes()
 
 
 @pytest.mark.parametrize("name", GRAPHS)

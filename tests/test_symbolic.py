@@ -44,6 +44,7 @@ CASES = [
     ("lap2d_64", "MMD"), ("lap2d_17", "MMD"), ("diff3d_10", "MMD"), ("st27_8", "MMD"),
     ("rand_300", "MMD"), ("rand_1000", "MMD"), ("rand_2000_sparse", "MMD"),
     ("lap2d_64", "RCM"), ("diff3d_10", "RCM"), ("rand_1000", "RCM"),
+    ("lap2d_64", "AMD"), ("st27_8", "AMD"), ("rand_2000_sparse", "AMD"),
 ]
 
 
@@ -53,8 +54,8 @@ def test_chain_from_reference_ordering(name, ordering):
     ri = G[name + "_rowind"]
     n = len(cp) - 1
     A = sp.DistSparseMatrix(n, cp, ri, np.ones(len(ri)), baseval=0)
-    if ordering == "MMD":
-        invp = G[name + "_mmd_invp"]
+    if ordering in ("MMD", "AMD"):
+        invp = G[name + "_%s_invp" % ordering.lower()]
         perm0 = np.argsort(invp) + 1
     else:
         perm0 = G[name + "_rcm_perm"]
