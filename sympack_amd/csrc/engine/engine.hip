@@ -536,6 +536,20 @@ void Engine::ensure_solve_capacity(int nrhs) {
   cap_nrhs_ = nrhs;
 }
 
+template <typename T, bool BWD>
+void Engine::launch_sdiag(int bucket, unsigned g, const Task4* t, const T* arena, const T* Y, T* X, int nrhs) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const T* dinv = (const T*)d_dinv_;
+  switch (bucket) {
+    case 1: dev::k_sdiag2<T, BWD, 1><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+    case 2: dev::k_sdiag2<T, BWD, 2><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+    case 4: dev::k_sdiag2<T, BWD, 4><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+    case 6: dev::k_sdiag2<T, BWD, 6><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+    case 10: dev::k_sdiag2<T, BWD, 10><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+    default: throw std::logic_error("bad diagonal-block bucket");
+  }
+}
+
 template <typename T>
 void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
@@ -546,7 +560,9 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
     const unsigned g = (unsigned)r.count;
     launch_begin(r.kind, r.count);
     if (r.kind == 3) {
-      if (ldl)
+      if (r.aux > 0)
+        launch_sdiag<T, false>(r.aux, g, sd + r.first, arena, Y, X, nrhs);
+      else if (ldl)
         dev::k_sdiag_fwd<T, true><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
       else
         dev::k_sdiag_fwd<T, false><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
@@ -566,7 +582,9 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
     if (r.kind == 5) {
       dev::k_soff_bwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, nrhs);
     } else {
-      if (ldl)
+      if (r.aux > 0)
+        launch_sdiag<T, true>(r.aux, g, sd + r.first, arena, X, X, nrhs);
+      else if (ldl)
         dev::k_sdiag_bwd<T, true><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, X, nrhs);
       else
         dev::k_sdiag_bwd<T, false><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, X, nrhs);

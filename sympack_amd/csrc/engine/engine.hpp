@@ -93,6 +93,8 @@ class Engine {
   void factor_sequence(T* arena);
   template <typename T>
   void launch_diag(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
+  template <typename T, bool BWD>
+  void launch_sdiag(int bucket, unsigned g, const Task4* t, const T* arena, const T* Y, T* X, int nrhs);
   template <typename T>
   void launch_ptrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
   template <typename T>

@@ -730,6 +730,100 @@ __global__ __launch_bounds__(NT) void k_sdiag_fwd(const SnInfo* __restrict__ sn,
   }
 }
 
+// Diagonal-block solve of a supernode of width <= 160 from the factor's
+// inverted 16x16 diagonal tiles (forward_update / _tri_solve,
+// symPACKMatrix2D.hpp:1486-1510, 1659-1707):
+//   forward   x_t = inv(L_tt) (y_t - sum_{u<t} L_tu x_u)
+//   backward  x_t = inv(L_tt)^T (y_t - sum_{u>t} L_ut^T x_u)
+// for 16-row blocks t.  The off-diagonal tiles of L_II and the inverse tiles
+// are staged in LDS once; right-hand sides go through LDS in chunks of 16,
+// thread (a, q) owning row a of the current block and right-hand side q.
+// LDL: unit inverse tiles, D applied by k_scale_d.  Task {sn, inverse-tile
+// base, -, -}; Y may alias X (backward sweep, in place).
+template <typename T, bool BWD, int NBM>
+__global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
+                                               const Task4* __restrict__ tasks,
+                                               const T* __restrict__ arena, const T* __restrict__ dinv,
+                                               const T* Y, T* X, int nrhs) {
+  constexpr int NTL = NBM * (NBM + 1) / 2;
+  __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
+  __shared__ T xs[NBM * SUB][SUB + 1];
+  __shared__ T ss[SUB][SUB + 1];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int w = s.w;
+  const int nbt = (w + SUB - 1) / SUB;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int tid = threadIdx.x;
+  {
+    const int r = tid & 15, c = tid >> 4;
+    const T* Dv = dinv + (int64_t)t.b * (SUB * SUB);
+    T v[NTL];
+#pragma unroll
+    for (int U = 0; U < NBM; ++U)
+#pragma unroll
+      for (int J = 0; J <= U; ++J) {
+        if (U == J) {
+          v[tslot(U, J)] = Dv[(int64_t)min(U, nbt - 1) * (SUB * SUB) + c * SUB + r];
+        } else {
+          const int row = min(SUB * U + r, w - 1), col = min(SUB * J + c, w - 1);
+          v[tslot(U, J)] = P[(int64_t)col * ld + row];
+        }
+      }
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) keep(v[q]);
+#pragma unroll
+    for (int U = 0; U < NBM; ++U)
+#pragma unroll
+      for (int J = 0; J <= U; ++J) {
+        const int row = SUB * U + r, col = SUB * J + c;
+        const bool in = (U == J) ? (U < nbt) : (row < w && col < w);
+        tl[tslot(U, J) * TSZ + c * TSTR + r] = in ? v[tslot(U, J)] : T(0);
+      }
+  }
+  const int a = tid >> 4, q = tid & 15;
+  const int64_t base = (int64_t)s.fc * nrhs;
+  for (int j0 = 0; j0 < nrhs; j0 += SUB) {
+    const int jc = min(SUB, nrhs - j0);
+    __syncthreads();  // tiles staged / previous chunk stored
+    for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
+      const int i = idx >> 4, j = idx & 15;
+      xs[i][j] = (i < w && j < jc) ? Y[base + (int64_t)i * nrhs + j0 + j] : T(0);
+    }
+    __syncthreads();
+    for (int step = 0; step < nbt; ++step) {
+      const int tb = BWD ? nbt - 1 - step : step;
+      T acc = xs[SUB * tb + a][q];
+      if (!BWD) {
+        for (int u = 0; u < tb; ++u) {
+          const T* L = tl + tslot(tb, u) * TSZ + a;  // L_tu[a][c] at c*TSTR
+#pragma unroll
+          for (int c = 0; c < SUB; ++c) acc -= L[c * TSTR] * xs[SUB * u + c][q];
+        }
+      } else {
+        for (int u = tb + 1; u < nbt; ++u) {
+          const T* L = tl + tslot(u, tb) * TSZ + a * TSTR;  // L_ut[c][a] at a*TSTR + c
+#pragma unroll
+          for (int c = 0; c < SUB; ++c) acc -= L[c] * xs[SUB * u + c][q];
+        }
+      }
+      ss[a][q] = acc;
+      __syncthreads();
+      T x = T(0);
+      const T* Li = tl + tslot(tb, tb) * TSZ;
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) x += (BWD ? Li[a * TSTR + c] : Li[c * TSTR + a]) * ss[c][q];
+      xs[SUB * tb + a][q] = x;
+      __syncthreads();
+    }
+    for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
+      const int i = idx >> 4, j = idx & 15;
+      if (i < w && j < jc) X[base + (int64_t)i * nrhs + j0 + j] = xs[i][j];
+    }
+  }
+}
+
 // Off-diagonal tile (rows rt*64.., columns kt*64.. of the panel below the
 // diagonal block).  Task {sn, row_tile, k_tile, -}.
 template <typename T>

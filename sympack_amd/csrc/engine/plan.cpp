@@ -122,6 +122,14 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       if (keyed[q].first == keyed[q + 1].first) p.amap[keyed[q].second] = -1;
   }
 
+  // inverse diagonal tiles: ceil(w/16) per supernode, contiguous, so block t
+  // of step st is tile inv_base + 10 st + t
+  p.inv_base.assign(ns, 0);
+  for (Int I = 0; I < ns; ++I) {
+    p.inv_base[I] = p.inv_tiles;
+    p.inv_tiles += ceil_div(p.sn[I].w, kSub);
+  }
+
   // ---- factorization task lists, level by level ----
   std::vector<std::vector<Int>> by_level(p.nlevels);
   for (Int I = 0; I < ns; ++I) by_level[p.level[I]].push_back(I);
@@ -142,8 +150,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
         int nb = (int)ceil_div(bs, kSub);
         nbmax = std::max(nbmax, nb);
         int64_t rows_below = si.r - c0 - bs;
-        int32_t inv_off = (int32_t)p.inv_tiles;
-        p.inv_tiles += nb;
+        int32_t inv_off = (int32_t)(p.inv_base[I] + (int64_t)st * (kStep / kSub));
         p.potrf_tasks.push_back({I, st, inv_off, 0});
         for (int64_t rt = 0; rt < ceil_div(rows_below, kTrsmRows); ++rt)
           p.trsm_tasks.push_back({I, st, (int32_t)rt, inv_off});
@@ -189,7 +196,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   for (int L = 0; L < p.nlevels; ++L) {
     const auto& sns = by_level[L];
     dfirst[L] = (int64_t)p.sdiag_tasks.size();
-    for (Int I : sns) p.sdiag_tasks.push_back({I, 0, 0, 0});
+    for (Int I : sns) p.sdiag_tasks.push_back({I, (int32_t)p.inv_base[I], 0, 0});
     ofirst[L] = (int64_t)p.soff_tasks.size();
     for (Int I : sns) {
       int64_t m = p.sn[I].r - p.sn[I].w;
@@ -200,14 +207,24 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     ocount[L] = (int64_t)p.soff_tasks.size() - ofirst[L];
   }
   // forward: levels bottom-up, diagonal solves then off-diagonal updates.
+  std::vector<int> dbucket(p.nlevels, 1);
   for (int L = 0; L < p.nlevels; ++L) {
-    p.fwd_seq.push_back({3, dfirst[L], (int64_t)by_level[L].size()});
+    int wmax = 1;
+    for (Int I : by_level[L]) wmax = std::max<int>(wmax, p.sn[I].w);
+    dbucket[L] = wmax > kStep ? 0 : diag_bucket((int)ceil_div(wmax, kSub));
+  }
+  for (int L = 0; L < p.nlevels; ++L) {
+    LaunchRange d{3, dfirst[L], (int64_t)by_level[L].size()};
+    d.aux = dbucket[L];
+    p.fwd_seq.push_back(d);
     if (ocount[L]) p.fwd_seq.push_back({4, ofirst[L], ocount[L]});
   }
   // backward: levels top-down, off-diagonal gathers then diagonal solves.
   for (int L = p.nlevels - 1; L >= 0; --L) {
     if (ocount[L]) p.bwd_seq.push_back({5, ofirst[L], ocount[L]});
-    p.bwd_seq.push_back({6, dfirst[L], (int64_t)by_level[L].size()});
+    LaunchRange d{6, dfirst[L], (int64_t)by_level[L].size()};
+    d.aux = dbucket[L];
+    p.bwd_seq.push_back(d);
   }
 }
 

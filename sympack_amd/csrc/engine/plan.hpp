@@ -69,13 +69,14 @@ struct Plan {
   std::vector<Task4> potrf_tasks;  // {sn, step, inverse-tile offset, 0}
   std::vector<Task4> trsm_tasks;   // {sn, step, row_tile, inverse-tile offset}
   int64_t inv_tiles = 0;           // 16x16 inverse diagonal tiles written by the factor
+  std::vector<int64_t> inv_base;   // per supernode: first inverse tile (tile t of 16-column block t)
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
   std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}
   std::vector<LaunchRange> factor_seq;
   // solve
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
   std::vector<Task4> soff_tasks;   // {sn, row_tile, k_tile, 0}: off-diagonal panel tiles
-  std::vector<LaunchRange> fwd_seq, bwd_seq;
+  std::vector<LaunchRange> fwd_seq, bwd_seq;  // kind 3/6 aux: diag_bucket of the level, 0 = w > kStep
   // algorithmic counts
   double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
   int64_t schur_entries = 0;       // lower-triangle entries scattered
