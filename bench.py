@@ -29,6 +29,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def build_problem(k):
     import sympack_amd as sp
     n, cp, ri, v = sp.matrices.diffusion_3d_7pt(k, seed=0)
@@ -97,6 +101,68 @@ def precision_leg(k, nrhs, reps):
     return out
 
 
+def ldl_leg(k, nrhs, reps):
+    """Static LDL^T of the shifted 7-point Laplacian L_h - 0.3 I on a k^3 grid
+    (symmetric indefinite): FP64 LDL^T against FP32 LDL^T + FP64 iterative
+    refinement, nrhs uniform[-1,1] right-hand sides resident in HBM.  Forward
+    difference against the FP64 factorization's solution and backward errors
+    of both."""
+    import numpy as np
+    import torch
+
+    import sympack_amd as sp
+    from oracle import numeric as onum
+
+    n, cp, ri, v = sp.matrices.shifted_laplacian_3d_7pt(k, 0.3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    dev = torch.device("cuda:0")
+    vals = torch.from_numpy(A.nzval).to(dev)
+    rng = np.random.default_rng(2)
+    B = rng.uniform(-1.0, 1.0, (n, nrhs))
+    b = torch.from_numpy(np.ascontiguousarray(B.ravel(order="F"))).to(dev)
+    out, X = {}, {}
+    for prec in ("FP64", "FP32_IR"):
+        rp = sp.RelaxationParameters()
+        rp.SetMaxSize(150)
+        M = sp.symPACKMatrix2D()
+        M.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", relax=rp, precision=prec,
+                                 ir_tol=1e-12))
+        M.SymbolicFactorization(A)
+        torch.cuda.synchronize()
+        M.DistributeMatrixDevice(vals.data_ptr())
+        M.Factorize()
+        M.SolveDevice(b.data_ptr(), nrhs)
+        torch.cuda.synchronize()
+        tf, ts = [], []
+        for _ in range(reps):
+            a = time.perf_counter()
+            M.Factorize()
+            c = time.perf_counter()
+            M.SolveDevice(b.data_ptr(), nrhs)
+            torch.cuda.synchronize()
+            d = time.perf_counter()
+            tf.append((c - a) * 1e3)
+            ts.append((d - c) * 1e3)
+        x = torch.empty(n * nrhs, dtype=torch.float64, device=dev)
+        M.GetSolutionDevice(x.data_ptr(), nrhs)
+        X[prec] = x.cpu().numpy().reshape(nrhs, n).T
+        st = M.stats()
+        key = "fp64" if prec == "FP64" else "fp32_ir"
+        out[key + "_factor_ms"] = round(float(np.median(tf)), 3)
+        out[key + "_solve_ms"] = round(float(np.median(ts)), 3)
+        out[key + "_factor_gflops"] = round(st["flops"] / (np.median(tf) * 1e-3) / 1e9, 1)
+        out[key + "_backward_error"] = onum.backward_error(A, X[prec], B)
+        if prec == "FP32_IR":
+            out["ir_iterations"] = st["ir_iterations"]
+        out["flops_sum_cc2"] = st["flops"]
+        del M
+    out["forward_diff_fp32_ir_vs_fp64"] = float(np.linalg.norm(X["FP32_IR"] - X["FP64"]) /
+                                                np.linalg.norm(X["FP64"]))
+    out["workload"] = ("shifted 3D 7-point Laplacian L_h - 0.3 I, %d^3 (n=%d), NDBOX, static LDL^T, "
+                       "nrhs=%d uniform[-1,1], median of %d factor+solve runs" % (k, n, nrhs, reps))
+    return out
+
+
 def serving_leg(n_requests, rate, precision, slots, seed=0):
     """Request mix through the serving layer: new factors of 2D/3D
     random-coefficient Laplacians (n log-uniform in [2e3, 2e5]) and follow-up
@@ -159,6 +225,9 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32+IR vs FP64 comparison on the 60^3 27-point stencil")
     ap.add_argument("--precision-k", type=int, default=60)
+    ap.add_argument("--no-ldl", action="store_true",
+                    help="skip the static LDL^T leg on the 80^3 shifted Laplacian")
+    ap.add_argument("--ldl-k", type=int, default=80)
     ap.add_argument("--serving", type=int, default=0, metavar="N",
                     help="also run N requests of the serving mix (FP64 and FP32+IR)")
     ap.add_argument("--serving-rate", type=float, default=200.0)
@@ -174,6 +243,7 @@ def main():
 
     import sympack_amd as sp
 
+    progress("building %d^3 diffusion matrix" % args.k)
     A = build_problem(args.k)
     n = A.size
     rp = sp.RelaxationParameters()
@@ -197,6 +267,7 @@ def main():
     st = M.stats()
     flops = st["flops"]
 
+    progress("warmup")
     for _ in range(args.warmup):
         M.Factorize()
         M.SolveDevice(b.data_ptr(), 1)
@@ -246,6 +317,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        progress("cpu baseline (oracle, one core)")
         S = onum.Structure.of(M)
         a = time.perf_counter()
         info, F = onum.factor(S, A)
@@ -258,9 +330,13 @@ def main():
                          "by oracle/supernodal.c on 1 host core: %.1f s; one solve %.3f s" %
                          (args.k, t_cf, t_cs)}
 
+    progress("precision leg")
     prec = None if args.no_precision else precision_leg(args.precision_k, 16, 3)
+    progress("ldl leg")
+    ldl = None if args.no_ldl else ldl_leg(args.ldl_k, 64, 3)
     serving = None
     if args.serving:
+        progress("serving leg")
         serving = [serving_leg(args.serving, args.serving_rate, p, args.serving_slots)
                    for p in ("FP64", "FP32_IR")]
 
@@ -305,6 +381,7 @@ def main():
                      "avg_launch_ms": schur_avg_ms},
         "cpu_baseline": cpu,
         "precision_fp32_ir_vs_fp64": prec,
+        "ldl_indefinite": ldl,
         "serving": serving,
     }
     print(json.dumps(out))
