@@ -55,7 +55,8 @@ typedef struct spk_options {
   int maxsize;        /* maximum supernode width, 0 = unlimited (driver default) */
   double zrelax0, zrelax1, zrelax2; /* 0.8, 0.1, 0.05 */
   int precision;      /* SPK_PREC_*; default FP64 */
-  int ir_max_iter;    /* refinement steps for SPK_PREC_FP32_IR (default 30) */
+  int ir_max_iter;    /* refinement steps for SPK_PREC_FP32_IR (default 30): preconditioned
+                         conjugate gradients for LL^T, fixed-point refinement for LDL^T */
   double ir_tol;      /* normwise backward-error target (default 1e-12) */
   const int* perm;    /* SPK_ORDER_USER: perm[new] = old, any base (symPACKOptions.perm) */
   int verbose;
@@ -75,6 +76,8 @@ typedef struct spk_stats {
   double factor_ms, solve_ms;     /* wall time of the last Factorize / Solve */
   int ir_iterations;              /* last Solve */
   double backward_error;          /* last Solve (FP32+IR), -1 if not computed */
+  double fp32_shift;              /* diagonal shift of the equilibrated matrix in the last
+                                     FP32 factor (0 unless its Cholesky broke down) */
 } spk_stats;
 
 /* Per-kernel-kind event timings (profile mode).  kind: 0 diagonal factor,

@@ -55,6 +55,7 @@ class SpkStats(ctypes.Structure):
         ("solve_ms", ctypes.c_double),
         ("ir_iterations", ctypes.c_int),
         ("backward_error", ctypes.c_double),
+        ("fp32_shift", ctypes.c_double),
     ]
 
 

@@ -88,6 +88,10 @@ void Engine::free_device() {
   dfree(d_x_);
   dfree(d_r_);
   dfree(d_red_);
+  dfree(d_z_);
+  dfree(d_p_);
+  dfree(d_q_);
+  dfree(d_s_);
   cap_nrhs_ = 0;
 }
 
@@ -105,7 +109,8 @@ int64_t Engine::device_bytes(int nrhs) const {
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
   b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
-  b += (int64_t)n_ * std::max(1, nrhs) * (2 * elem + 3 * 8) + 16 * std::max(1, nrhs);
+  b += (int64_t)n_ * std::max(1, nrhs) * (2 * elem + (opt_.precision == 0 ? 3 : 6) * 8) +
+       48 * std::max(1, nrhs);
   return b;
 }
 
@@ -211,19 +216,24 @@ void Engine::distribute(const double* nzval, bool on_device) {
   HIP_OK(hipStreamSynchronize(stream_));
   have_values_ = true;
   factored_ = false;
+  fp32_level_ = 0;
 }
 
-// Equilibrated assembly: entry (r,c) scaled by s_r s_c.
+// Equilibrated assembly: entry (r,c) scaled by s_r s_c, plus `shift` on the
+// diagonal (only after an FP32 breakdown of an SPD matrix).
 template <typename T>
 __global__ void k_assemble_scaled(T* __restrict__ arena, const int64_t* __restrict__ amap,
                                   const double* __restrict__ vals, const int32_t* __restrict__ rowind,
                                   const int32_t* __restrict__ colidx, const double* __restrict__ s,
-                                  int64_t nnz) {
+                                  int64_t nnz, double shift) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t dst = amap[e];
     if (dst < 0) continue;
-    arena[dst] = (T)(vals[e] * s[rowind[e]] * s[colidx[e]]);
+    const int r = rowind[e], c = colidx[e];
+    double v = vals[e] * s[r] * s[c];
+    if (r == c) v += shift;
+    arena[dst] = (T)v;
   }
 }
 
@@ -234,7 +244,7 @@ void Engine::assemble(T* arena) {
     dev::k_assemble<T><<<grid_for(nnzA_), 256, 0, stream_>>>(arena, d_amap_, d_vals_, nullptr, nnzA_);
   else
     k_assemble_scaled<T><<<grid_for(nnzA_), 256, 0, stream_>>>(arena, d_amap_, d_vals_, d_arowind_,
-                                                               d_acolidx_, d_scale_, nnzA_);
+                                                               d_acolidx_, d_scale_, nnzA_, fp32_shift_);
   HIP_OK(hipGetLastError());
 }
 
@@ -244,16 +254,6 @@ void Engine::launch_begin(int kind, int64_t grid) {
   else
     kind_stats[kind].launches++;
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
-  if (capturing_) {
-    // events owned by the graph, recorded by every replay
-    hipEvent_t a, b;
-    HIP_OK(hipEventCreate(&a));
-    HIP_OK(hipEventCreate(&b));
-    cap_events_.push_back({kind, {a, b}});
-    cap_grids_.push_back(grid);
-    HIP_OK(hipEventRecord(a, stream_));
-    return;
-  }
   ev_grid_.push_back(grid);
   while (ev_pool_.size() < ev_next_ + 2) {
     hipEvent_t e;
@@ -268,10 +268,6 @@ void Engine::launch_begin(int kind, int64_t grid) {
 
 void Engine::launch_end(int kind) {
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
-  if (capturing_) {
-    HIP_OK(hipEventRecord(cap_events_.back().second.second, stream_));
-    return;
-  }
   HIP_OK(hipEventRecord(ev_open_.back().second.second, stream_));
 }
 
@@ -294,9 +290,11 @@ void Engine::destroy_graphs() {
 
 // Runs `body` (a sequence of launches on stream_) through the graph slot g:
 // captured on first use or when the key changes, replayed afterwards.
+// Profile mode launches eagerly: events recorded inside a graph cannot be
+// timed with hipEventElapsedTime.
 template <typename F>
 void Engine::run_graph(Graph& g, int64_t key, F&& body) {
-  if (!use_graphs_) {
+  if (!use_graphs_ || opt_.profile) {
     body();
     return;
   }
@@ -319,13 +317,7 @@ void Engine::run_graph(Graph& g, int64_t key, F&& body) {
         (void)hipEventDestroy(e.second.second);
       }
       cap_events_.clear();
-      (void)hipGetLastError();
-      if (!opt_.profile) throw;
-      // event records that cannot be captured: profile eagerly from now on
-      use_graphs_ = false;
-      destroy_graphs();
-      body();
-      return;
+      throw;
     }
     capturing_ = false;
     HIP_OK(hipStreamEndCapture(stream_, &graph));
@@ -499,10 +491,25 @@ int Engine::factorize() {
       factor_sequence(d_arena64_);
     });
   } else {
-    run_graph(fgraph_, 2, [&] {
-      assemble(d_arena32_);
-      factor_sequence(d_arena32_);
-    });
+    // FP32 factor of the equilibrated matrix.  An SPD matrix whose FP32
+    // Cholesky breaks down is factored again with the diagonal of S A S
+    // shifted by 1e-6, 1e-4, 1e-2: the shifted factor is still a
+    // preconditioner for the FP64 refinement, which solves the unshifted
+    // system.
+    for (int level = fp32_level_;; ++level) {
+      fp32_level_ = level;
+      fp32_shift_ = level == 0 ? 0.0 : std::pow(10.0, -8.0 + 2.0 * level);
+      run_graph(fgraph_, 2 + 16 * level, [&] {
+        assemble(d_arena32_);
+        factor_sequence(d_arena32_);
+      });
+      if (opt_.ldl || level == 3) break;
+      int inf = 0;
+      HIP_OK(hipMemcpyAsync(&inf, d_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      if (inf == big) break;
+      HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
+    }
   }
   int info = 0;
   HIP_OK(hipMemcpyAsync(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -525,6 +532,10 @@ void Engine::ensure_solve_capacity(int nrhs) {
   dfree(d_x_);
   dfree(d_r_);
   dfree(d_red_);
+  dfree(d_z_);
+  dfree(d_p_);
+  dfree(d_q_);
+  dfree(d_s_);
   size_t tsz = opt_.precision == 0 ? sizeof(double) : sizeof(float);
   size_t cnt = (size_t)n_ * nrhs;
   HIP_OK(hipMalloc(&d_Y_, std::max<size_t>(1, cnt) * tsz));
@@ -533,6 +544,12 @@ void Engine::ensure_solve_capacity(int nrhs) {
   d_x_ = dalloc<double>(cnt);
   d_r_ = dalloc<double>(cnt);
   d_red_ = dalloc<double>(2 * (size_t)nrhs);
+  if (opt_.precision == 1) {
+    d_z_ = dalloc<double>(cnt);
+    d_p_ = dalloc<double>(cnt);
+    d_q_ = dalloc<double>(cnt);
+    d_s_ = dalloc<double>(4 * (size_t)nrhs);
+  }
   cap_nrhs_ = nrhs;
 }
 
@@ -629,6 +646,75 @@ __global__ void k_colmax2(const double* __restrict__ r, const double* __restrict
   }
 }
 
+// y += A x for the symmetric matrix held as its lower CSC (column-major
+// n x nrhs vectors, original numbering).
+__global__ void k_symv(const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+                       const double* __restrict__ val, const double* __restrict__ x,
+                       double* __restrict__ y, int n, int nrhs) {
+  int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  for (int j = 0; j < nrhs; ++j) {
+    const double* xj = x + (int64_t)j * n;
+    double* yj = y + (int64_t)j * n;
+    const double xc = xj[c];
+    double acc = 0.0;
+    for (int64_t p = colptr[c] + lane; p < colptr[c + 1]; p += 64) {
+      const int row = rowind[p];
+      const double v = val[p];
+      if (row != c) {
+        unsafeAtomicAdd(yj + row, v * xc);
+        acc += v * xj[row];
+      } else {
+        acc += v * xc;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) unsafeAtomicAdd(yj + c, acc);
+  }
+}
+
+// out[j] = sum_i a[i,j] b[i,j], one block per column (fixed summation order).
+__global__ void k_coldot(const double* __restrict__ a, const double* __restrict__ b, int n,
+                         double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += a[(int64_t)j * n + i] * b[(int64_t)j * n + i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sh[0];
+}
+
+// CG step: alpha_j = rho_j / (p_j . q_j); x += alpha p; r -= alpha q.
+__global__ void k_pcg_xr(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                         const double* __restrict__ q, const double* __restrict__ rho,
+                         const double* __restrict__ pq, int n, int nrhs) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n);
+    const double alpha = pq[j] != 0.0 ? rho[j] / pq[j] : 0.0;
+    x[idx] += alpha * p[idx];
+    r[idx] -= alpha * q[idx];
+  }
+}
+
+// CG step: beta_j = rho_new_j / rho_j; p = z + beta p.
+__global__ void k_pcg_p(double* __restrict__ p, const double* __restrict__ z,
+                        const double* __restrict__ rho_new, const double* __restrict__ rho, int n,
+                        int nrhs) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n);
+    const double beta = rho[j] != 0.0 ? rho_new[j] / rho[j] : 0.0;
+    p[idx] = z[idx] + beta * p[idx];
+  }
+}
+
 // One refinement step: d = S (SAS)^{-1} S r in the factor's precision,
 // x += d, r = b - A x and the per-RHS norms, all on the device.
 template <typename T>
@@ -639,6 +725,49 @@ void Engine::ir_step(T* Y, T* X, int nrhs) {
   dev::k_scatter_sol<T><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
   residual(d_b_, d_x_, d_r_, nrhs);
   k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+  HIP_OK(hipGetLastError());
+}
+
+// z = S (L L^T)^{-1} S r with the FP32 factor of S A S (FP64 in and out).
+void Engine::precond(const double* r, double* z, int nrhs) {
+  const unsigned g = grid_for((int64_t)n_ * nrhs);
+  float* Y = (float*)d_Y_;
+  float* X = (float*)d_X_;
+  dev::k_gather_rhs<float><<<g, 256, 0, stream_>>>(r, Y, d_perm0_, n_, nrhs, d_scale_);
+  solve_sequence<float>(d_arena32_, Y, X, nrhs);
+  dev::k_scatter_sol<float><<<g, 256, 0, stream_>>>(X, z, d_perm0_, n_, nrhs, d_scale_, 0);
+  HIP_OK(hipGetLastError());
+}
+
+// One preconditioned-CG iteration for every right-hand side (FP64 vectors).
+// Scalars live on the device: rho = d_s_[0,nrhs), p.q = [nrhs,2nrhs),
+// rho_new = [2nrhs,3nrhs).
+void Engine::pcg_iteration(int nrhs) {
+  const int64_t cnt = (int64_t)n_ * nrhs;
+  const unsigned g = grid_for(cnt);
+  double* rho = d_s_;
+  double* pq = d_s_ + nrhs;
+  double* rho_new = d_s_ + 2 * nrhs;
+  dev::k_fill_zero<double><<<g, 256, 0, stream_>>>(d_q_, cnt);
+  const int cols_per_block = 4;
+  k_symv<<<(unsigned)((n_ + cols_per_block - 1) / cols_per_block), 64 * cols_per_block, 0, stream_>>>(
+      d_acolptr_, d_arowind_, d_vals_, d_p_, d_q_, n_, nrhs);
+  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_p_, d_q_, n_, pq);
+  k_pcg_xr<<<g, 256, 0, stream_>>>(d_x_, d_r_, d_p_, d_q_, rho, pq, n_, nrhs);
+  precond(d_r_, d_z_, nrhs);
+  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_z_, n_, rho_new);
+  k_pcg_p<<<g, 256, 0, stream_>>>(d_p_, d_z_, rho_new, rho, n_, nrhs);
+  dev::k_copy<<<1, 256, 0, stream_>>>(rho_new, rho, (int64_t)nrhs);
+  k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+  HIP_OK(hipGetLastError());
+}
+
+// (Re)start: r given; z = M^{-1} r; p = z; rho = r.z.
+void Engine::pcg_restart(int nrhs) {
+  const int64_t cnt = (int64_t)n_ * nrhs;
+  precond(d_r_, d_z_, nrhs);
+  dev::k_copy<<<grid_for(cnt), 256, 0, stream_>>>(d_z_, d_p_, cnt);
+  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_z_, n_, d_s_);
   HIP_OK(hipGetLastError());
 }
 
@@ -663,22 +792,14 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
       HIP_OK(hipGetLastError());
     });
   } else {
-    // FP64 iterative refinement around the FP32 factor of S A S
-    // (S = diag(s)): x_0 = 0, r_0 = b; d = S (SAS)^{-1} S r; x += d;
-    // r = b - A x in FP64; stop when the normwise backward error is met.
-    float* Y = (float*)d_Y_;
-    float* X = (float*)d_X_;
-    dev::k_fill_zero<double><<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_x_, (int64_t)cnt);
-    HIP_OK(hipMemcpyAsync(d_r_, d_b_, cnt * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    // ||b||_inf per rhs (x = 0 at this point)
+    // FP64 refinement around the FP32 factor of S A S (S = diag(s)).
+    // SPD (LL^T): preconditioned conjugate gradients in FP64 with
+    // M^{-1} = S (LL^T)^{-1} S, converging also where kappa(A) u_32 ~ 1;
+    // static LDL^T (indefinite): fixed-point refinement x += M^{-1}(b - A x).
+    // Both stop at the normwise backward error ir_tol, checked on the true
+    // residual b - A x.
     std::vector<double> red(2 * (size_t)nrhs), bnorm(nrhs);
-    k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_b_, d_x_, n_, d_red_);
-    HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    HIP_OK(hipStreamSynchronize(stream_));
-    for (int j = 0; j < nrhs; ++j) bnorm[j] = red[2 * j];
-    rep.converged = 0;
-    for (int it = 0; it < std::max(1, opt_.ir_max_iter); ++it) {
-      run_graph(solve_graph(4 * (int64_t)nrhs + 1), 4 * (int64_t)nrhs + 1, [&] { ir_step<float>(Y, X, nrhs); });
+    auto read_berr = [&]() {
       HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
       HIP_OK(hipStreamSynchronize(stream_));
       double berr = 0.0;
@@ -686,11 +807,48 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
         double den = normA_inf_ * red[2 * j + 1] + bnorm[j];
         berr = std::max(berr, den > 0 ? red[2 * j] / den : 0.0);
       }
-      rep.iterations = it + 1;
-      rep.backward_error = berr;
-      if (berr <= opt_.ir_tol) {
-        rep.converged = 1;
-        break;
+      return berr;
+    };
+    dev::k_fill_zero<double><<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_x_, (int64_t)cnt);
+    dev::k_copy<<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_b_, d_r_, (int64_t)cnt);
+    k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_b_, d_x_, n_, d_red_);
+    read_berr();
+    for (int j = 0; j < nrhs; ++j) bnorm[j] = red[2 * j];
+    rep.converged = 0;
+    const int maxit = std::max(1, opt_.ir_max_iter);
+    if (!opt_.ldl) {
+      const int64_t kit = 4 * (int64_t)nrhs + 2, kst = 4 * (int64_t)nrhs + 3;
+      run_graph(solve_graph(kst), kst, [&] { pcg_restart(nrhs); });
+      for (int it = 0; it < maxit; ++it) {
+        run_graph(solve_graph(kit), kit, [&] { pcg_iteration(nrhs); });
+        double berr = read_berr();
+        rep.iterations = it + 1;
+        if (berr <= opt_.ir_tol || it + 1 == maxit) {
+          residual(d_b_, d_x_, d_r_, nrhs);  // the true residual decides
+          k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+          berr = read_berr();
+          rep.backward_error = berr;
+          if (berr <= opt_.ir_tol) {
+            rep.converged = 1;
+            break;
+          }
+          run_graph(solve_graph(kst), kst, [&] { pcg_restart(nrhs); });
+        }
+        rep.backward_error = berr;
+      }
+    } else {
+      float* Y = (float*)d_Y_;
+      float* X = (float*)d_X_;
+      const int64_t kir = 4 * (int64_t)nrhs + 1;
+      for (int it = 0; it < maxit; ++it) {
+        run_graph(solve_graph(kir), kir, [&] { ir_step<float>(Y, X, nrhs); });
+        const double berr = read_berr();
+        rep.iterations = it + 1;
+        rep.backward_error = berr;
+        if (berr <= opt_.ir_tol) {
+          rep.converged = 1;
+          break;
+        }
       }
     }
   }

@@ -102,6 +102,9 @@ class Engine {
   template <typename T>
   void assemble(T* arena);
   void residual(const double* b, const double* x, double* r, int nrhs);
+  void precond(const double* r, double* z, int nrhs);
+  void pcg_iteration(int nrhs);
+  void pcg_restart(int nrhs);
   void launch_begin(int kind, int64_t grid = 0);
   void launch_end(int kind);
 
@@ -168,6 +171,14 @@ class Engine {
   double* d_x_ = nullptr;
   double* d_r_ = nullptr;
   double* d_red_ = nullptr;
+  double* d_z_ = nullptr;  // preconditioned CG vectors (FP32 factor mode)
+  double* d_p_ = nullptr;
+  double* d_q_ = nullptr;
+  double* d_s_ = nullptr;  // per-RHS CG scalars
+ public:
+  double fp32_shift_ = 0.0;  // diagonal shift of S A S used by the last FP32 factor
+ private:
+  int fp32_level_ = 0;
   void ensure_solve_capacity(int nrhs);
   double normA_inf_ = 0.0;
   // profiling events

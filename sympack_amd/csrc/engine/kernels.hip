@@ -229,6 +229,8 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     T a[SUB];
 #pragma unroll
     for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
+    // every wave reads the diagonal tile, which wave 0 overwrites below
+    __syncthreads();
     T yv[SUB];  // 1/L_jj (LL) or 1/D_j (LDL)
 #pragma unroll
     for (int j = 0; j < SUB; ++j) {

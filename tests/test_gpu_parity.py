@@ -203,7 +203,7 @@ def test_graph_replays_match_eager(prec):
         assert G.kernel_stats(k)[0] == E.kernel_stats(k)[0]
 
 
-def test_profiled_graph_records_kernel_times():
+def test_profile_mode_records_kernel_times():
     n, cp, ri, v = M.diffusion_3d_7pt(10, seed=22)
     A = sp.DistSparseMatrix(n, cp, ri, v)
     rp = sp.RelaxationParameters()
@@ -222,3 +222,71 @@ def test_profiled_graph_records_kernel_times():
     assert launches > 0 and ms > 0
     kinds, grids, times = Mx.launch_trace()
     assert len(kinds) == launches and set(kinds) == {2} and np.all(times > 0)
+
+
+def test_fp32_pcg_ill_conditioned_27pt():
+    # diagonal = row abs-sum * (1 + 1e-6): kappa ~ 1e6-1e7, beyond plain
+    # fixed-point refinement with an FP32 factor
+    n, cp, ri, v = M.stencil_3d_27pt(24, seed=13)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision="FP32_IR", ir_tol=1e-12))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    rng = np.random.default_rng(14)
+    B = rng.standard_normal((n, 16))
+    Mx.Solve(B, 16)
+    X = Mx.GetSolution(16)
+    st = Mx.stats()
+    print("pcg iterations", st["ir_iterations"], "berr", st["backward_error"], flush=True)
+    assert st["backward_error"] <= 1e-12
+    assert onum.backward_error(A, X, B) <= 1e-12
+
+
+def test_fp32_nearly_singular_spd():
+    # graph Laplacian + 1e-10 I: SPD in FP64, at the edge of FP32
+    n, cp, ri, v = M.diffusion_3d_7pt(16, seed=15, shift=1e-10)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision="FP32_IR", ir_tol=1e-12,
+                              ir_max_iter=300))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    rng = np.random.default_rng(16)
+    B = rng.standard_normal((n, 2))
+    Mx.Solve(B, 2)
+    X = Mx.GetSolution(2)
+    st = Mx.stats()
+    print("shift", st["fp32_shift"], "iterations", st["ir_iterations"], "berr", st["backward_error"], flush=True)
+    assert onum.backward_error(A, X, B) <= 1e-12
+
+
+def test_ldl_48cube_refactorizations_match_oracle():
+    # regression: a read-after-write race on the LDS diagonal tile of the
+    # diagonal-block factor showed up here as backward errors ~1e-2 that
+    # changed from one factorization to the next
+    n, cp, ri, v = M.shifted_laplacian_3d_7pt(48, 0.3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", relax=rp))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    rng = np.random.default_rng(17)
+    B = rng.uniform(-1, 1, (n, 2))
+    S = onum.Structure.of(Mx)
+    info, F = onum.factor(S, A, True)
+    be_o = onum.backward_error(A, onum.solve(S, F, B, True), B)
+    for rep in range(3):
+        Mx.Factorize()
+        Mx.Solve(B, 2)
+        be = onum.backward_error(A, Mx.GetSolution(2), B)
+        print("refactor", rep, "berr %.2e oracle %.2e" % (be, be_o), flush=True)
+        assert be <= 10 * max(be_o, 1e-16)
