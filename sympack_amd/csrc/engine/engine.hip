@@ -47,7 +47,7 @@ inline unsigned grid_for(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
 }
-enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF };
+enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM };
 }  // namespace
 
 Engine::Engine() { kind_stats.resize(8); }
@@ -105,7 +105,7 @@ int64_t Engine::device_bytes(int nrhs) const {
   const Plan& p = plan_;
   int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.amap) + vb(p.perm0) +
               vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) +
-              vb(p.sdiag_tasks) + vb(p.soff_tasks);
+              vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
   b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
@@ -164,7 +164,7 @@ void Engine::ensure_device() {
   d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
   d_tasks_[TA_SCHUR] = dupload(plan_.schur_tasks, s);
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
-  d_tasks_[TA_SOFF] = dupload(plan_.soff_tasks, s);
+  d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
   d_info_ = dalloc<int>(1);
   d_acolptr_ = dupload(h_colptr_, s);
   d_arowind_ = dupload(h_rowind_, s);
@@ -572,7 +572,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
   const bool ldl = opt_.ldl != 0;
   const Task4* sd = (const Task4*)d_tasks_[TA_SDIAG];
-  const Task4* so = (const Task4*)d_tasks_[TA_SOFF];
+  const Task4* sm = (const Task4*)d_tasks_[TA_SOFFM];
   for (const LaunchRange& r : plan_.fwd_seq) {
     const unsigned g = (unsigned)r.count;
     launch_begin(r.kind, r.count);
@@ -584,7 +584,8 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
       else
         dev::k_sdiag_fwd<T, false><<<g, 256, 0, stream_>>>(sn, sd + r.first, arena, Y, X, nrhs);
     } else {
-      dev::k_soff_fwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, Y, nrhs);
+      const dim3 grid((unsigned)r.count, (unsigned)((nrhs + 15) / 16));
+      dev::k_soffm_fwd<T><<<grid, 256, 0, stream_>>>(sn, sm + r.first, arena, d_lindx0_, X, Y, nrhs);
     }
     launch_end(r.kind);
   }
@@ -597,7 +598,8 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
     const unsigned g = (unsigned)r.count;
     launch_begin(r.kind, r.count);
     if (r.kind == 5) {
-      dev::k_soff_bwd<T><<<g, 256, 0, stream_>>>(sn, so + r.first, arena, d_lindx0_, X, nrhs);
+      const dim3 grid((unsigned)r.count, (unsigned)((nrhs + 15) / 16));
+      dev::k_soffm_bwd<T><<<grid, 256, 0, stream_>>>(sn, sm + r.first, arena, d_lindx0_, X, nrhs);
     } else {
       if (r.aux > 0)
         launch_sdiag<T, true>(r.aux, g, sd + r.first, arena, X, X, nrhs);

@@ -153,7 +153,7 @@ class Engine {
   int32_t* d_rel_ = nullptr;
   int64_t* d_amap_ = nullptr;
   int32_t* d_perm0_ = nullptr;
-  void* d_tasks_[8] = {nullptr};  // potrf, trsm, inpanel, schur, sdiag, soff
+  void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm
   int* d_info_ = nullptr;
   int64_t* d_acolptr_ = nullptr;
   int32_t* d_arowind_ = nullptr;

@@ -18,7 +18,7 @@
 //                FP32 MFMA 16x16x4, scattered straight from registers into the
 //                target panels through precomputed relative indices with
 //                hardware atomics (blockCell_t::update, :1180-1466).
-//  k_sdiag_fwd / k_soff_fwd / k_scale_d / k_soff_bwd / k_sdiag_bwd
+//  k_sdiag2 / k_sdiag_fwd / k_soffm_fwd / k_scale_d / k_soffm_bwd / k_sdiag_bwd
 //                supernodal forward/backward solve (forward_update[_contrib],
 //                back_update[_contrib], :1486-1797).
 #include <hip/hip_runtime.h>
@@ -165,12 +165,12 @@ __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 +
 // blockCellLDL_t::factorize, :2096-2116, lapack.hpp:206-617).
 // Right-looking over 16-column block columns J:
 //  1. every wave factors block column J lane-per-row: lanes 0-15 hold the
-//     16 rows of the diagonal tile (each wave keeps its own copy, so pivots
-//     are broadcast with readlane and need no barrier), lanes 16-63 hold 48
-//     rows below it; wave 1 also forms the inverse of the diagonal tile,
-//     which the panel TRSM uses;
+//     16 rows of the diagonal tile (each wave keeps its own copy, so the
+//     pivot columns are broadcast through a per-wave LDS buffer without
+//     barriers), lanes 16-63 hold 48 rows below it;
 //  2. the trailing lower tiles get A_IK -= L_IJ L_KJ^T (LDL: L_IJ D_J L_KJ^T)
-//     on MFMA 16x16x4, tiles spread over the 4 waves.
+//     on MFMA 16x16x4, spread over waves 0-2, while wave 3 inverts the
+//     diagonal tile for the panel TRSM.
 // Rows/columns beyond bs are padded with an identity block.  Task
 // {sn, step, inverse-tile offset, -}; info receives the smallest failing
 // pivot column (1-based, permuted order).
@@ -184,6 +184,7 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
   constexpr int NTL = NBM * (NBM + 1) / 2;
   __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
   __shared__ T dsh[SUB];
+  __shared__ __attribute__((aligned(16))) T colb[NT / 64][SUB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
   const int c0 = t.b * kStep;
@@ -232,9 +233,19 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     // every wave reads the diagonal tile, which wave 0 overwrites below
     __syncthreads();
     T yv[SUB];  // 1/L_jj (LL) or 1/D_j (LDL)
+    // column j of the diagonal rows (still unscaled) is broadcast through
+    // this wave's LDS buffer: LDS operations of one wave complete in order,
+    // so no barrier is needed between the write and the reads
+    T* __restrict__ cb = colb[wave];
 #pragma unroll
     for (int j = 0; j < SUB; ++j) {
-      T piv = readlane_t(a[j], j);
+      if (diag_lane) cb[lane] = a[j];
+      compiler_fence();
+      T piv = cb[j];
+      T u[SUB];
+#pragma unroll
+      for (int k = j + 1; k < SUB; ++k) u[k] = cb[k];
+      compiler_fence();
       const bool bad = LDL ? (piv == T(0) || piv != piv) : !(piv > T(0));
       if (bad) {
         if (SUB * J + j < bs && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + SUB * J + j + 1);
@@ -245,14 +256,14 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
         yv[j] = inv;
         const T l = a[j] * inv;
 #pragma unroll
-        for (int k = j + 1; k < SUB; ++k) a[k] -= l * readlane_t(a[j], k);
+        for (int k = j + 1; k < SUB; ++k) a[k] -= l * u[k];
         a[j] = (lane == j) ? piv : l;
       } else {
         const T y = rsqrt_nr(piv);
         yv[j] = y;
         const T t2 = a[j] * y * y;
 #pragma unroll
-        for (int k = j + 1; k < SUB; ++k) a[k] -= t2 * readlane_t(a[j], k);
+        for (int k = j + 1; k < SUB; ++k) a[k] -= t2 * u[k];
         a[j] = a[j] * y;
       }
     }
@@ -262,15 +273,18 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
       for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (diag_lane && c > lane) ? T(0) : a[c];
     }
     if (LDL && wave == 0 && diag_lane) dsh[lane] = a[lane];
-    if (wave == 1) {
-      // inverse of the diagonal tile, column `lane` per lane (forward
-      // substitution L z = e_lane); LDL: unit lower triangle
+    __syncthreads();
+    if (wave == 3) {
+      // inverse of the factored diagonal tile (now in LDS), column `lane`
+      // per lane: forward substitution L z = e_lane (LDL: unit diagonal);
+      // overlaps the trailing update of waves 0-2
+      const T* Lt = tl + tslot(J, J) * TSZ;
       T z[SUB];
 #pragma unroll
       for (int i = 0; i < SUB; ++i) {
         T acc = (lane == i) ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc -= readlane_t(a[k], i) * z[k];
+        for (int k = 0; k < i; ++k) acc -= Lt[k * TSTR + i] * z[k];
         z[i] = LDL ? acc : acc * yv[i];
       }
       if (diag_lane) {
@@ -279,11 +293,10 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
         for (int i = 0; i < SUB; ++i) out[i] = z[i];
       }
     }
-    __syncthreads();
     // ---- 2. trailing update of tiles (I,K), J < K <= I < nb ----
     const int m = nb - J - 1;
     const int ntr = m * (m + 1) / 2;
-    for (int q = wave; q < ntr; q += NT / 64) {
+    for (int q = wave; wave < 3 && q < ntr; q += 3) {
       // decode q -> (K, I): columns K = J+1.., rows I = K..nb-1
       int K = J + 1, rem = q, len = m;
       while (rem >= len) {
@@ -404,15 +417,21 @@ __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
 #pragma unroll
   for (int K = 0; K < NBM; ++K) {
     if (K < nb) {
+      // two accumulators (even / odd J) halve the dependent MFMA chain
+      v4 y2 = v4{0, 0, 0, 0};
 #pragma unroll
       for (int J = 0; J < K; ++J) {
         const T* L = tl + tslot(K, J) * TSZ + (lane & 15);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
           const int kp = Mfma<T>::row(lane, kk);
-          y[K] = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y[K]);
+          if (J & 1)
+            y2 = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y2);
+          else
+            y[K] = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y[K]);
         }
       }
+      y[K] += y2;
       const T* Li = tl + tslot(K, K) * TSZ + (lane & 15);
       v4 z = v4{0, 0, 0, 0};
 #pragma unroll
@@ -788,6 +807,12 @@ __global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
   const int64_t base = (int64_t)s.fc * nrhs;
   for (int j0 = 0; j0 < nrhs; j0 += SUB) {
     const int jc = min(SUB, nrhs - j0);
+    // jc2 = right-hand sides per 16-thread row group (power of two >= jc);
+    // the nsp = 16 / jc2 threads sharing a right-hand side split every dot
+    // product by column and reduce with shuffles inside the group
+    const int jc2 = jc > 8 ? 16 : jc > 4 ? 8 : jc > 2 ? 4 : jc > 1 ? 2 : 1;
+    const int nsp = SUB / jc2;
+    const int jq = q % jc2, sp = q / jc2;
     __syncthreads();  // tiles staged / previous chunk stored
     for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
       const int i = idx >> 4, j = idx & 15;
@@ -796,27 +821,26 @@ __global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
     __syncthreads();
     for (int step = 0; step < nbt; ++step) {
       const int tb = BWD ? nbt - 1 - step : step;
-      T acc = xs[SUB * tb + a][q];
+      T acc = T(0);
       if (!BWD) {
         for (int u = 0; u < tb; ++u) {
           const T* L = tl + tslot(tb, u) * TSZ + a;  // L_tu[a][c] at c*TSTR
-#pragma unroll
-          for (int c = 0; c < SUB; ++c) acc -= L[c * TSTR] * xs[SUB * u + c][q];
+          for (int c = sp; c < SUB; c += nsp) acc -= L[c * TSTR] * xs[SUB * u + c][jq];
         }
       } else {
         for (int u = tb + 1; u < nbt; ++u) {
           const T* L = tl + tslot(u, tb) * TSZ + a * TSTR;  // L_ut[c][a] at a*TSTR + c
-#pragma unroll
-          for (int c = 0; c < SUB; ++c) acc -= L[c] * xs[SUB * u + c][q];
+          for (int c = sp; c < SUB; c += nsp) acc -= L[c] * xs[SUB * u + c][jq];
         }
       }
-      ss[a][q] = acc;
+      for (int off = jc2; off < SUB; off <<= 1) acc += __shfl_xor(acc, off, SUB);
+      if (sp == 0) ss[a][jq] = acc + xs[SUB * tb + a][jq];
       __syncthreads();
       T x = T(0);
       const T* Li = tl + tslot(tb, tb) * TSZ;
-#pragma unroll
-      for (int c = 0; c < SUB; ++c) x += (BWD ? Li[a * TSTR + c] : Li[c * TSTR + a]) * ss[c][q];
-      xs[SUB * tb + a][q] = x;
+      for (int c = sp; c < SUB; c += nsp) x += (BWD ? Li[a * TSTR + c] : Li[c * TSTR + a]) * ss[c][jq];
+      for (int off = jc2; off < SUB; off <<= 1) x += __shfl_xor(x, off, SUB);
+      if (sp == 0) xs[SUB * tb + a][jq] = x;
       __syncthreads();
     }
     for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
@@ -826,84 +850,106 @@ __global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
   }
 }
 
-// Off-diagonal tile (rows rt*64.., columns kt*64.. of the panel below the
-// diagonal block).  Task {sn, row_tile, k_tile, -}.
+// Multi-RHS off-diagonal solve updates on MFMA (used for nrhs >= 4), one
+// workgroup per (64 panel rows below the diagonal block, 16 right-hand
+// sides); wave q owns rows 16q..16q+15.  Task {sn, row tile, -, -},
+// blockIdx.y = right-hand-side chunk.
+// Forward:  Y[row(a)] -= sum_k L(a,k) X_I(k)      (one atomic per output)
 template <typename T>
-__global__ __launch_bounds__(NT) void k_soff_fwd(const SnInfo* __restrict__ sn,
-                                                  const Task4* __restrict__ tasks,
-                                                  const T* __restrict__ arena,
-                                                  const int32_t* __restrict__ lindx0,
-                                                  const T* __restrict__ X, T* __restrict__ Y,
-                                                  int nrhs) {
-  __shared__ T Ps[NB][TILE + 1];  // Ps[k][a]
-  __shared__ T Xs[NB][JC + 1];
+__global__ __launch_bounds__(NT) void k_soffm_fwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   const int32_t* __restrict__ lindx0,
+                                                   const T* __restrict__ X, T* __restrict__ Y,
+                                                   int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
-  const int tid = threadIdx.x;
-  const T* __restrict__ P = arena + s.off;
-  const int64_t ld = s.r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = s.r - s.w;
-  const int a0 = t.b * TILE, k0 = t.c * NB;
-  const int na = min(TILE, m - a0), nk = min(NB, s.w - k0);
-  for (int idx = tid; idx < NB * TILE; idx += NT) {
-    int a = idx % TILE, k = idx / TILE;
-    Ps[k][a] = (a < na && k < nk) ? P[(int64_t)(k0 + k) * ld + s.w + a0 + a] : T(0);
+  const int a0 = t.b * TILE + 16 * wave;
+  if (a0 >= m) return;
+  const int j0 = blockIdx.y * SUB;
+  const int mi = lane & 15, g = lane >> 4;
+  const int ac = min(a0 + mi, m - 1);
+  const T amask = T(a0 + mi < m);
+  const int jn = j0 + mi;
+  const int jc = min(jn, nrhs - 1);
+  const T jmask = T(jn < nrhs);
+  const T* __restrict__ Pa = arena + s.off + s.w + ac;      // L(a, k) at Pa[k * ld]
+  const T* __restrict__ Xk = X + (int64_t)s.fc * nrhs + jc;  // X_I(k, j) at Xk[k * nrhs]
+  const int64_t ld = s.r;
+  const int w = s.w;
+  v4 acc = v4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < w; k0 += 32) {
+    T av[8], bv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = k0 + 4 * q + g;
+      const int kc = min(k, w - 1);
+      const T km = T(k < w);
+      av[q] = Pa[(int64_t)kc * ld] * (amask * km);
+      bv[q] = Xk[(int64_t)kc * nrhs] * jmask;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc = Mfma<T>::mma(av[q], bv[q], acc);
   }
-  const int a = tid % TILE, g = tid / TILE;
-  const int row = (a < na) ? lindx0[s.lx + s.w + a0 + a] : 0;
-  for (int j0 = 0; j0 < nrhs; j0 += JC) {
-    const int jc = min(JC, nrhs - j0);
-    __syncthreads();
-    for (int idx = tid; idx < nk * jc; idx += NT) {
-      int k = idx / jc, jj = idx % jc;
-      Xs[k][jj] = X[(int64_t)(s.fc + k0 + k) * nrhs + j0 + jj];
-    }
-    __syncthreads();
-    if (a < na) {
-      for (int jj = g; jj < jc; jj += 4) {
-        T acc = T(0);
-        for (int k = 0; k < nk; ++k) acc += Ps[k][a] * Xs[k][jj];
-        atomic_add(Y + (int64_t)row * nrhs + j0 + jj, -acc);
-      }
-    }
+  if (jn >= nrhs) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int a = a0 + Mfma<T>::row(lane, v);
+    if (a < m) atomic_add(Y + (int64_t)lindx0[s.lx + s.w + a] * nrhs + jn, -acc[v]);
   }
 }
 
+// Backward:  X_I(k) -= sum_{a in tile} L(a,k) Y[row(a)]   (partial sums over
+// 64 rows, atomics into X_I).  The reduction index is permuted so that lane
+// (m, g) reads rows 16g..16g+15 of one column: one cache line per lane.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_soff_bwd(const SnInfo* __restrict__ sn,
-                                                  const Task4* __restrict__ tasks,
-                                                  const T* __restrict__ arena,
-                                                  const int32_t* __restrict__ lindx0,
-                                                  T* __restrict__ X, int nrhs) {
-  __shared__ T Ps[NB][TILE + 1];  // Ps[k][a]
-  __shared__ T Xs[TILE][JC + 1];  // gathered solution rows
+__global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   const int32_t* __restrict__ lindx0,
+                                                   T* __restrict__ X, int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
-  const int tid = threadIdx.x;
-  const T* __restrict__ P = arena + s.off;
-  const int64_t ld = s.r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = s.r - s.w;
-  const int a0 = t.b * TILE, k0 = t.c * NB;
-  const int na = min(TILE, m - a0), nk = min(NB, s.w - k0);
-  for (int idx = tid; idx < NB * TILE; idx += NT) {
-    int a = idx % TILE, k = idx / TILE;
-    Ps[k][a] = (a < na && k < nk) ? P[(int64_t)(k0 + k) * ld + s.w + a0 + a] : T(0);
+  const int a0 = t.b * TILE;
+  const int j0 = blockIdx.y * SUB;
+  const int mi = lane & 15, g = lane >> 4;
+  const int jn = j0 + mi;
+  const int jc = min(jn, nrhs - 1);
+  const T jmask = T(jn < nrhs);
+  const int64_t ld = s.r;
+  // B operand (shared by every column block): Y rows of this tile
+  T bv[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int a = a0 + 16 * g + st;
+    const int acl = min(a, m - 1);
+    const int row = lindx0[s.lx + s.w + acl];
+    bv[st] = X[(int64_t)row * nrhs + jc] * (jmask * T(a < m));
   }
-  const int k = tid % NB, g = tid / NB;
-  for (int j0 = 0; j0 < nrhs; j0 += JC) {
-    const int jc = min(JC, nrhs - j0);
-    __syncthreads();
-    for (int idx = tid; idx < na * jc; idx += NT) {
-      int a = idx / jc, jj = idx % jc;
-      int row = lindx0[s.lx + s.w + a0 + a];
-      Xs[a][jj] = X[(int64_t)row * nrhs + j0 + jj];
+  const int nkb = (s.w + 15) >> 4;
+  for (int kb = wave; kb < nkb; kb += NT / 64) {
+    const int kcol = min(kb * 16 + mi, s.w - 1);
+    const T* __restrict__ Pc = arena + s.off + (int64_t)kcol * ld + s.w;
+    T av[16];
+#pragma unroll
+    for (int st = 0; st < 16; ++st) {
+      const int a = min(a0 + 16 * g + st, m - 1);
+      av[st] = Pc[a];
     }
-    __syncthreads();
-    if (k < nk) {
-      for (int jj = g; jj < jc; jj += 4) {
-        T acc = T(0);
-        for (int a = 0; a < na; ++a) acc += Ps[k][a] * Xs[a][jj];
-        atomic_add(X + (int64_t)(s.fc + k0 + k) * nrhs + j0 + jj, -acc);
+    v4 acc = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[st], bv[st], acc);
+    if (jn < nrhs) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = kb * 16 + Mfma<T>::row(lane, v);
+        if (k < s.w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, -acc[v]);
       }
     }
   }

@@ -190,21 +190,19 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   }
 
   // ---- solve task lists ----
-  // Per level: one diagonal task per supernode (bwd_tasks holds {sn}) and
+  // Per level: one diagonal task per supernode ({sn, inverse-tile base}) and
   // one off-diagonal task per 64-row tile of the panel below the diagonal.
-  std::vector<int64_t> dfirst(p.nlevels), ofirst(p.nlevels), ocount(p.nlevels);
+  std::vector<int64_t> dfirst(p.nlevels), mfirst(p.nlevels), mcount(p.nlevels);
   for (int L = 0; L < p.nlevels; ++L) {
     const auto& sns = by_level[L];
     dfirst[L] = (int64_t)p.sdiag_tasks.size();
     for (Int I : sns) p.sdiag_tasks.push_back({I, (int32_t)p.inv_base[I], 0, 0});
-    ofirst[L] = (int64_t)p.soff_tasks.size();
+    mfirst[L] = (int64_t)p.soffm_tasks.size();
     for (Int I : sns) {
       int64_t m = p.sn[I].r - p.sn[I].w;
-      int64_t nkt = ceil_div(p.sn[I].w, kNB);
-      for (int64_t rt = 0; rt < ceil_div(m, kTile); ++rt)
-        for (int64_t kt = 0; kt < nkt; ++kt) p.soff_tasks.push_back({I, (int32_t)rt, (int32_t)kt, 0});
+      for (int64_t rt = 0; rt < ceil_div(m, kTile); ++rt) p.soffm_tasks.push_back({I, (int32_t)rt, 0, 0});
     }
-    ocount[L] = (int64_t)p.soff_tasks.size() - ofirst[L];
+    mcount[L] = (int64_t)p.soffm_tasks.size() - mfirst[L];
   }
   // forward: levels bottom-up, diagonal solves then off-diagonal updates.
   std::vector<int> dbucket(p.nlevels, 1);
@@ -217,11 +215,11 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     LaunchRange d{3, dfirst[L], (int64_t)by_level[L].size()};
     d.aux = dbucket[L];
     p.fwd_seq.push_back(d);
-    if (ocount[L]) p.fwd_seq.push_back({4, ofirst[L], ocount[L]});
+    if (mcount[L]) p.fwd_seq.push_back({4, mfirst[L], mcount[L]});
   }
   // backward: levels top-down, off-diagonal gathers then diagonal solves.
   for (int L = p.nlevels - 1; L >= 0; --L) {
-    if (ocount[L]) p.bwd_seq.push_back({5, ofirst[L], ocount[L]});
+    if (mcount[L]) p.bwd_seq.push_back({5, mfirst[L], mcount[L]});
     LaunchRange d{6, dfirst[L], (int64_t)by_level[L].size()};
     d.aux = dbucket[L];
     p.bwd_seq.push_back(d);

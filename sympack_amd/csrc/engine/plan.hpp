@@ -47,7 +47,7 @@ struct LaunchRange {
                      // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
-  int aux = 0;       // kinds 0 and 7: diagonal-block bucket (diag_bucket)
+  int aux = 0;       // kinds 0, 7, 3, 6: diagonal-block bucket (diag_bucket)
 };
 
 struct Plan {
@@ -75,7 +75,7 @@ struct Plan {
   std::vector<LaunchRange> factor_seq;
   // solve
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
-  std::vector<Task4> soff_tasks;   // {sn, row_tile, k_tile, 0}: off-diagonal panel tiles
+  std::vector<Task4> soffm_tasks;  // {sn, row_tile, 0, 0}: 64-row tiles of the off-diagonal panel
   std::vector<LaunchRange> fwd_seq, bwd_seq;  // kind 3/6 aux: diag_bucket of the level, 0 = w > kStep
   // algorithmic counts
   double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
