@@ -326,6 +326,13 @@ def main():
     schur_flops_per_launch = st["schur_flops"] / max(1, schur_launches_per_step)
     achieved_tflops = schur_flops_per_launch / (schur_avg_ms * 1e-3) / 1e12 if sms > 0 else None
 
+    # HBM bytes per Schur launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes (scaled by a plain copy of known size) committed under profiles/
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "schur_update_traffic_48cube.json")
+    if args.k == 48 and os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu_baseline:
         progress("cpu baseline (oracle, one core)")
@@ -390,7 +397,8 @@ def main():
                      "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
                      "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_MFMA_PEAK_TFLOPS, 4) if achieved_tflops else None,
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": "profiles/schur_update_hbm_traffic_48cube.md" if traffic else None,
                      "flops_per_launch": schur_flops_per_launch,
                      "avg_launch_ms": schur_avg_ms},
         "cpu_baseline": cpu,
