@@ -18,7 +18,7 @@
 //                FP32 MFMA 16x16x4, scattered straight from registers into the
 //                target panels through precomputed relative indices with
 //                hardware atomics (blockCell_t::update, :1180-1466).
-//  k_sdiag2 / k_sdiag_fwd / k_soffm_fwd / k_scale_d / k_soffm_bwd / k_sdiag_bwd
+//  k_sdiag3 / k_sdiag_fwd / k_soffm_fwd / k_scale_d / k_soffm_bwd / k_sdiag_bwd
 //                supernodal forward/backward solve (forward_update[_contrib],
 //                back_update[_contrib], :1486-1797).
 #include <hip/hip_runtime.h>
@@ -239,12 +239,16 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     T* __restrict__ cb = colb[wave];
 #pragma unroll
     for (int j = 0; j < SUB; ++j) {
+      // the pivot and the next row's entry, which form the dependency chain
+      // from one pivot to the next, come by readlane; the rest of the
+      // column through LDS, off that chain
       if (diag_lane) cb[lane] = a[j];
       compiler_fence();
-      T piv = cb[j];
+      T piv = readlane_t(a[j], j);
       T u[SUB];
+      if (j + 1 < SUB) u[j + 1] = readlane_t(a[j], j + 1);
 #pragma unroll
-      for (int k = j + 1; k < SUB; ++k) u[k] = cb[k];
+      for (int k = j + 2; k < SUB; ++k) u[k] = cb[k];
       compiler_fence();
       const bool bad = LDL ? (piv == T(0) || piv != piv) : !(piv > T(0));
       if (bad) {
@@ -534,6 +538,35 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     }
   };
   fetch(0);
+  // MODE 1: scatter offsets of the 16 outputs of this lane, loaded while
+  // the first operand chunk is in flight (rel is defined for rows a >= the
+  // first row of b's target group: clamped, unconditional loads)
+  int64_t off[2][4][2];
+  if (MODE == 1) {
+    const int m = s.r - s.w;
+    int64_t tm[2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
+        off[bi][v][0] = tcol[s.upd + bc];
+        tm[bi][v] = tmap[s.upd + bc];
+      }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
+        const int64_t tc = off[bi][v][0];
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) {
+          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
+          const int ac = min(max(aoff, bc), m - 1);
+          off[bi][v][aj] = tc + rel[tm[bi][v] + ac];
+        }
+      }
+  }
   for (int kc = 0; kc < K; kc += KC) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -598,44 +631,6 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         }
   } else {
     const int m = s.r - s.w;
-    int64_t tc[2][4], tm[2][4];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
-        const int bc = min(boff, m - 1);
-        tc[bi][v] = tcol[s.upd + bc];
-        tm[bi][v] = tmap[s.upd + bc];
-      }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        keep(tc[bi][v]);
-        keep(tm[bi][v]);
-      }
-    int32_t rl[2][4][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int boff = tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
-        const int bc = min(boff, m - 1);
-#pragma unroll
-        for (int aj = 0; aj < 2; ++aj) {
-          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
-          // rel is defined for rows a >= the first row of b's target group
-          const int ac = min(max(aoff, bc), m - 1);
-          rl[bi][v][aj] = rel[tm[bi][v] + ac];
-        }
-      }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int aj = 0; aj < 2; ++aj) keep(rl[bi][v][aj]);
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -644,8 +639,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 #pragma unroll
         for (int aj = 0; aj < 2; ++aj) {
           const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
-          if (boff < m && aoff < m && aoff >= boff)
-            atomic_add(arena + tc[bi][v] + rl[bi][v][aj], -acc[bi][aj][v]);
+          if (boff < m && aoff < m && aoff >= boff) atomic_add(arena + off[bi][v][aj], -acc[bi][aj][v]);
         }
       }
   }
@@ -751,32 +745,29 @@ __global__ __launch_bounds__(NT) void k_sdiag_fwd(const SnInfo* __restrict__ sn,
   }
 }
 
-// Diagonal-block solve of a supernode of width <= 160 from the factor's
-// inverted 16x16 diagonal tiles (forward_update / _tri_solve,
-// symPACKMatrix2D.hpp:1486-1510, 1659-1707):
+// Diagonal-block solve on MFMA (w <= 160), the panel-TRSM recurrence with
+// right-hand sides in place of panel rows: for 16-row blocks t
 //   forward   x_t = inv(L_tt) (y_t - sum_{u<t} L_tu x_u)
 //   backward  x_t = inv(L_tt)^T (y_t - sum_{u>t} L_ut^T x_u)
-// for 16-row blocks t.  The off-diagonal tiles of L_II and the inverse tiles
-// are staged in LDS once; right-hand sides go through LDS in chunks of 16,
-// thread (a, q) owning row a of the current block and right-hand side q.
+// Each wave owns 16 right-hand sides (x^T in MFMA accumulators, whose
+// layout is the B operand of the next product); a workgroup covers 64.
 // LDL: unit inverse tiles, D applied by k_scale_d.  Task {sn, inverse-tile
-// base, -, -}; Y may alias X (backward sweep, in place).
+// base, -, -}, blockIdx.y = chunk of 64 right-hand sides; Y may alias X.
 template <typename T, bool BWD, int NBM>
-__global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
+__global__ __launch_bounds__(NT) void k_sdiag3(const SnInfo* __restrict__ sn,
                                                const Task4* __restrict__ tasks,
                                                const T* __restrict__ arena, const T* __restrict__ dinv,
                                                const T* Y, T* X, int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
   constexpr int NTL = NBM * (NBM + 1) / 2;
   __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
-  __shared__ T xs[NBM * SUB][SUB + 1];
-  __shared__ T ss[SUB][SUB + 1];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
   const int w = s.w;
   const int nbt = (w + SUB - 1) / SUB;
   const T* __restrict__ P = arena + s.off;
   const int64_t ld = s.r;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   {
     const int r = tid & 15, c = tid >> 4;
     const T* Dv = dinv + (int64_t)t.b * (SUB * SUB);
@@ -803,49 +794,64 @@ __global__ __launch_bounds__(NT) void k_sdiag2(const SnInfo* __restrict__ sn,
         tl[tslot(U, J) * TSZ + c * TSTR + r] = in ? v[tslot(U, J)] : T(0);
       }
   }
-  const int a = tid >> 4, q = tid & 15;
-  const int64_t base = (int64_t)s.fc * nrhs;
-  for (int j0 = 0; j0 < nrhs; j0 += SUB) {
-    const int jc = min(SUB, nrhs - j0);
-    // jc2 = right-hand sides per 16-thread row group (power of two >= jc);
-    // the nsp = 16 / jc2 threads sharing a right-hand side split every dot
-    // product by column and reduce with shuffles inside the group
-    const int jc2 = jc > 8 ? 16 : jc > 4 ? 8 : jc > 2 ? 4 : jc > 1 ? 2 : 1;
-    const int nsp = SUB / jc2;
-    const int jq = q % jc2, sp = q / jc2;
-    __syncthreads();  // tiles staged / previous chunk stored
-    for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
-      const int i = idx >> 4, j = idx & 15;
-      xs[i][j] = (i < w && j < jc) ? Y[base + (int64_t)i * nrhs + j0 + j] : T(0);
+  __syncthreads();
+  const int jb = blockIdx.y * 64 + 16 * wave;
+  if (jb >= nrhs) return;  // no barrier follows
+  const int j = jb + (lane & 15);
+  const int jcl = min(j, nrhs - 1);
+  const T jm = T(j < nrhs);
+  const int64_t base = (int64_t)s.fc * nrhs + jcl;
+  v4 x[NBM];
+#pragma unroll
+  for (int K = 0; K < NBM; ++K)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = min(SUB * K + Mfma<T>::row(lane, v), w - 1);
+      x[K][v] = Y[base + (int64_t)row * nrhs] * jm;
     }
-    __syncthreads();
-    for (int step = 0; step < nbt; ++step) {
-      const int tb = BWD ? nbt - 1 - step : step;
-      T acc = T(0);
-      if (!BWD) {
-        for (int u = 0; u < tb; ++u) {
-          const T* L = tl + tslot(tb, u) * TSZ + a;  // L_tu[a][c] at c*TSTR
-          for (int c = sp; c < SUB; c += nsp) acc -= L[c * TSTR] * xs[SUB * u + c][jq];
-        }
-      } else {
-        for (int u = tb + 1; u < nbt; ++u) {
-          const T* L = tl + tslot(u, tb) * TSZ + a * TSTR;  // L_ut[c][a] at a*TSTR + c
-          for (int c = sp; c < SUB; c += nsp) acc -= L[c] * xs[SUB * u + c][jq];
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) asm volatile("" : "+v"(x[K]));
+#pragma unroll
+  for (int Ks = 0; Ks < NBM; ++Ks) {
+    const int K = BWD ? NBM - 1 - Ks : Ks;
+    if (K < nbt) {
+      v4 x2 = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int J = 0; J < NBM; ++J) {
+        if (BWD ? (J > K && J < nbt) : (J < K)) {
+          const T* L = BWD ? tl + tslot(J, K) * TSZ + (lane & 15) * TSTR  // L_JK^T[m][k'] = L_JK[k'][m]
+                           : tl + tslot(K, J) * TSZ + (lane & 15);         // L_KJ[m][k']
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int kp = Mfma<T>::row(lane, kk);
+            const T a = BWD ? L[kp] : L[kp * TSTR];
+            if (J & 1)
+              x2 = Mfma<T>::mma(-a, x[J][kk], x2);
+            else
+              x[K] = Mfma<T>::mma(-a, x[J][kk], x[K]);
+          }
         }
       }
-      for (int off = jc2; off < SUB; off <<= 1) acc += __shfl_xor(acc, off, SUB);
-      if (sp == 0) ss[a][jq] = acc + xs[SUB * tb + a][jq];
-      __syncthreads();
-      T x = T(0);
-      const T* Li = tl + tslot(tb, tb) * TSZ;
-      for (int c = sp; c < SUB; c += nsp) x += (BWD ? Li[a * TSTR + c] : Li[c * TSTR + a]) * ss[c][jq];
-      for (int off = jc2; off < SUB; off <<= 1) x += __shfl_xor(x, off, SUB);
-      if (sp == 0) xs[SUB * tb + a][jq] = x;
-      __syncthreads();
+      x[K] += x2;
+      const T* Li = BWD ? tl + tslot(K, K) * TSZ + (lane & 15) * TSTR : tl + tslot(K, K) * TSZ + (lane & 15);
+      v4 z = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        z = Mfma<T>::mma(BWD ? Li[kp] : Li[kp * TSTR], x[K][kk], z);
+      }
+      x[K] = z;
     }
-    for (int idx = tid; idx < NBM * SUB * SUB; idx += NT) {
-      const int i = idx >> 4, j = idx & 15;
-      if (i < w && j < jc) X[base + (int64_t)i * nrhs + j0 + j] = xs[i][j];
+  }
+  if (j >= nrhs) return;
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) {
+    if (K < nbt) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = SUB * K + Mfma<T>::row(lane, v);
+        if (row < w) X[base + (int64_t)row * nrhs] = x[K][v];
+      }
     }
   }
 }

@@ -71,7 +71,7 @@ struct Plan {
   int64_t inv_tiles = 0;           // 16x16 inverse diagonal tiles written by the factor
   std::vector<int64_t> inv_base;   // per supernode: first inverse tile (tile t of 16-column block t)
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
-  std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}
+  std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}: 64x64 output tiles
   std::vector<LaunchRange> factor_seq;
   // solve
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
