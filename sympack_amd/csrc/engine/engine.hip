@@ -248,7 +248,8 @@ void Engine::assemble(T* arena) {
   HIP_OK(hipGetLastError());
 }
 
-void Engine::launch_begin(int kind, int64_t grid) {
+void Engine::launch_begin(int kind, int64_t grid, hipStream_t st) {
+  if (!st) st = stream_;
   if (capturing_)
     cap_kinds_.push_back(kind);
   else
@@ -262,13 +263,14 @@ void Engine::launch_begin(int kind, int64_t grid) {
   }
   hipEvent_t a = ev_pool_[ev_next_], b = ev_pool_[ev_next_ + 1];
   ev_next_ += 2;
-  HIP_OK(hipEventRecord(a, stream_));
+  HIP_OK(hipEventRecord(a, st));
   ev_open_.push_back({kind, {a, b}});
 }
 
-void Engine::launch_end(int kind) {
+void Engine::launch_end(int kind, hipStream_t st) {
+  if (!st) st = stream_;
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
-  HIP_OK(hipEventRecord(ev_open_.back().second.second, stream_));
+  HIP_OK(hipEventRecord(ev_open_.back().second.second, st));
 }
 
 void Engine::destroy_graph(Graph& g) {
@@ -443,7 +445,8 @@ void Engine::factor_sequence(T* arena) {
   const bool ldl = opt_.ldl != 0;
   for (const LaunchRange& r : plan_.factor_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind, r.count);
+    hipStream_t st = stream_;
+    launch_begin(r.kind, r.count, st);
     switch (r.kind) {
       case 0: {
         const Task4* t = (const Task4*)d_tasks_[TA_POTRF] + r.first;
@@ -458,23 +461,23 @@ void Engine::factor_sequence(T* arena) {
       case 1: {
         const Task4* t = (const Task4*)d_tasks_[TA_INPANEL] + r.first;
         if (ldl)
-          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
         else
-          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
         break;
       }
       case 2: {
         const Task4* t = (const Task4*)d_tasks_[TA_SCHUR] + r.first;
         if (ldl)
-          dev::k_update<T, 1, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+          dev::k_update<T, 1, true><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
         else
-          dev::k_update<T, 1, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_);
+          dev::k_update<T, 1, false><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
         break;
       }
       default:
         throw std::logic_error("bad factor launch kind");
     }
-    launch_end(r.kind);
+    launch_end(r.kind, st);
   }
   HIP_OK(hipGetLastError());
 }

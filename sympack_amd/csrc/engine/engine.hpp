@@ -105,8 +105,8 @@ class Engine {
   void precond(const double* r, double* z, int nrhs);
   void pcg_iteration(int nrhs);
   void pcg_restart(int nrhs);
-  void launch_begin(int kind, int64_t grid = 0);
-  void launch_end(int kind);
+  void launch_begin(int kind, int64_t grid = 0, hipStream_t st = nullptr);
+  void launch_end(int kind, hipStream_t st = nullptr);
 
   // captured launch sequences
   struct Graph {
@@ -140,6 +140,7 @@ class Engine {
   bool have_symbolic_ = false, have_values_ = false, factored_ = false, device_ready_ = false;
   void ensure_device();
   hipStream_t stream_ = nullptr;
+
   // caller CSC (0-based) kept for assembly and residuals
   Int n_ = 0;
   int64_t nnzA_ = 0;

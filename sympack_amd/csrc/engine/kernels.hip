@@ -478,14 +478,16 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
                                                 T* __restrict__ arena,
                                                 const int64_t* __restrict__ tcol,
                                                 const int64_t* __restrict__ tmap,
-                                                const int32_t* __restrict__ rel) {
+                                                const int32_t* __restrict__ rel, int64_t ntasks) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int KC = 32;
-  constexpr int LDS_LD = TILE + 16;
+  constexpr int LDS_LD = TILE + 1;  // odd pitch: the 4 k rows a wave reads hit different banks
   __shared__ T As[KC][LDS_LD];
   __shared__ T Bs[KC][LDS_LD];
   __shared__ T Dk[KC];
-  const Task4 t = tasks[blockIdx.x];
+  // grid-stride over the tasks (the launches use one workgroup per task)
+  for (int64_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
+  const Task4 t = tasks[task];
   const SnInfo s = sn[t.a];
   T* __restrict__ P = arena + s.off;
   const int64_t ld = s.r;
@@ -642,6 +644,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
           if (boff < m && aoff < m && aoff >= boff) atomic_add(arena + off[bi][v][aj], -acc[bi][aj][v]);
         }
       }
+  }
   }
 }
 

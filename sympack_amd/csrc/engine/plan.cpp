@@ -135,6 +135,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   for (Int I = 0; I < ns; ++I) by_level[p.level[I]].push_back(I);
   for (int L = 0; L < p.nlevels; ++L) {
     const auto& sns = by_level[L];
+    const size_t level_first = p.factor_seq.size();
     int smax = 0;
     for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kStep));
     for (int st = 0; st < smax; ++st) {
@@ -187,6 +188,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
     sr.count = (int64_t)p.schur_tasks.size() - sr.first;
     if (sr.count) p.factor_seq.push_back(sr);
+    for (size_t q = level_first; q < p.factor_seq.size(); ++q) p.factor_seq[q].level = L;
   }
 
   // ---- solve task lists ----

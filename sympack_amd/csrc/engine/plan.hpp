@@ -48,6 +48,7 @@ struct LaunchRange {
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
   int aux = 0;       // kinds 0, 7, 3, 6: diagonal-block bucket (diag_bucket)
+  int level = 0;     // etree level of the launch (factor sequence)
 };
 
 struct Plan {
