@@ -62,6 +62,10 @@ typedef struct spk_options {
   int verbose;
   int profile;        /* time every kernel launch with HIP events */
   int device;         /* HIP device ordinal */
+  int merge_updates;  /* 1 (default): a chain child's update outside its parent's columns is
+                         computed inside the parent's Schur update (one scatter instead of two)
+                         on etree levels with enough update tiles to fill the GPU; 2: on every
+                         level; 0: every supernode scatters its whole update */
 } spk_options;
 
 typedef struct spk_stats {
@@ -154,6 +158,11 @@ int spk_get_structure(spk_handle h, int64_t* xlindx /* nsuper+1 */, int32_t* lin
 int spk_get_cell_counts(spk_handle h, int64_t* ncells, int64_t* nblocks);
 int spk_get_cells(spk_handle h, int32_t* cell_K, int32_t* cell_I, int32_t* cell_nrows,
                   int32_t* cell_nblocks, int32_t* block_first_row, int32_t* block_nrows);
+/* Schur-update merging chosen by the plan (no counterpart in the reference,
+ * whose tasks update each target cell from one source supernode):
+ * deferred_into[I] = 0-based parent that computes the part of supernode I's
+ * update outside the parent's columns, -1 where I scatters its whole update. */
+int spk_get_update_merges(spk_handle h, int32_t* deferred_into /* nsuper */);
 /* Factor panel of supernode I (1-based): r x w column-major, structure rows
  * by supernode columns; LDL panels hold D on the diagonal. */
 int spk_get_panel_dims(spk_handle h, int32_t I, int32_t* nrows, int32_t* ncols);

@@ -72,6 +72,7 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   e.verbose = o.verbose;
   e.profile = o.profile;
   e.device = o.device;
+  e.merge_updates = o.merge_updates;
   return e;
 }
 }  // namespace
@@ -126,6 +127,7 @@ void spk_default_options(spk_options* o) {
   o->verbose = 0;
   o->profile = 0;
   o->device = g_device;
+  o->merge_updates = 1;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -181,6 +183,8 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         o->ir_tol = std::atof(val(1).c_str());
       } else if (k == "-ir_iter") {
         o->ir_max_iter = std::atoi(val(1).c_str());
+      } else if (k == "-merge") {
+        o->merge_updates = std::atoi(val(1).c_str());
       }
     }
     return 0;
@@ -426,6 +430,16 @@ int spk_get_structure(spk_handle h, int64_t* xlindx, int32_t* lindx) {
   if (xlindx) std::memcpy(xlindx, s.xlindx.data(), s.xlindx.size() * sizeof(int64_t));
   if (lindx) std::memcpy(lindx, s.lindx.data(), s.lindx.size() * sizeof(int32_t));
   return 0;
+}
+
+int spk_get_update_merges(spk_handle h, int32_t* deferred_into) {
+  if (!h || !deferred_into) return fail("null argument");
+  return guarded([&]() {
+    const spk::Plan& p = h->engine.plan();
+    if ((int)p.deferred_into.size() != p.nsuper) throw std::logic_error("no symbolic factorization");
+    std::copy(p.deferred_into.begin(), p.deferred_into.end(), deferred_into);
+    return 0;
+  });
 }
 
 int spk_get_cell_counts(spk_handle h, int64_t* ncells, int64_t* nblocks) {

@@ -70,6 +70,8 @@ void Engine::free_device() {
   dfree(d_tcol_);
   dfree(d_tmap_);
   dfree(d_rel_);
+  dfree(d_merges_);
+  dfree(d_mmap_);
   dfree(d_amap_);
   dfree(d_perm0_);
   for (auto& t : d_tasks_) dfree(t);
@@ -103,7 +105,7 @@ void Engine::init(const EngineOptions& o) {
 int64_t Engine::device_bytes(int nrhs) const {
   auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
   const Plan& p = plan_;
-  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.amap) + vb(p.perm0) +
+  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
               vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
@@ -130,7 +132,7 @@ void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) 
   have_symbolic_ = have_values_ = factored_ = device_ready_ = false;
   const Int* up = opt_.user_perm.empty() ? nullptr : opt_.user_perm.data();
   symbolic_factorization(n, colptr, rowind, baseval, opt_.ordering, opt_.relax, up, sym_);
-  build_plan(sym_, n, colptr, rowind, baseval, plan_);
+  build_plan(sym_, n, colptr, rowind, baseval, plan_, opt_.merge_updates);
   n_ = n;
   nnzA_ = colptr[n] - baseval;
   h_colptr_.assign(n + 1, 0);
@@ -157,6 +159,8 @@ void Engine::ensure_device() {
   d_tcol_ = dupload(plan_.tcol, s);
   d_tmap_ = dupload(plan_.tmap, s);
   d_rel_ = dupload(plan_.rel, s);
+  d_merges_ = dupload(plan_.merges, s);
+  d_mmap_ = dupload(plan_.mmap, s);
   d_amap_ = dupload(plan_.amap, s);
   d_perm0_ = dupload(plan_.perm0, s);
   d_tasks_[TA_POTRF] = dupload(plan_.potrf_tasks, s);
@@ -461,17 +465,17 @@ void Engine::factor_sequence(T* arena) {
       case 1: {
         const Task4* t = (const Task4*)d_tasks_[TA_INPANEL] + r.first;
         if (ldl)
-          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
+          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
         else
-          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
+          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
         break;
       }
       case 2: {
         const Task4* t = (const Task4*)d_tasks_[TA_SCHUR] + r.first;
         if (ldl)
-          dev::k_update<T, 1, true><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
+          dev::k_update<T, 1, true><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
         else
-          dev::k_update<T, 1, false><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, r.count);
+          dev::k_update<T, 1, false><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
         break;
       }
       default:

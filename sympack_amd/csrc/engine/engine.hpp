@@ -23,6 +23,7 @@ struct EngineOptions {
   int device = 0;
   int profile = 0;        // record HIP events around every launch
   int verbose = 0;
+  int merge_updates = 1;  // 0 off, 1 on levels that fill the GPU, 2 every qualifying chain child
   std::vector<Int> user_perm;
 };
 
@@ -152,6 +153,8 @@ class Engine {
   int64_t* d_tcol_ = nullptr;
   int64_t* d_tmap_ = nullptr;
   int32_t* d_rel_ = nullptr;
+  MergeInfo* d_merges_ = nullptr;
+  int32_t* d_mmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
   int32_t* d_perm0_ = nullptr;
   void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm

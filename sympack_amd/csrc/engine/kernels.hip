@@ -137,6 +137,20 @@ __device__ __forceinline__ float rcp_nr(float a) {
   y = y * (2.0f - a * y);
   return y;
 }
+// Reciprocal in three dependent operations after the hardware estimate:
+// with e = 1 - a y0, 1/a = y0 (1 + e + e^2 + ...), so y0 + y0 (e + e^2) leaves
+// a relative error of e^3.  The gfx950 f64 estimate has |e| < 2^-24 (measured
+// over 2^20 arguments spanning 2^-30..2^30), hence e^3 < 2^-72: full double
+// precision.  The f32 estimate is already within an ulp or two, one step.
+__device__ __forceinline__ double rcp_cubic(double a) {
+  const double y0 = __builtin_amdgcn_rcp(a);
+  const double e = fma(-a, y0, 1.0);
+  return fma(y0, fma(e, e, e), y0);
+}
+__device__ __forceinline__ float rcp_cubic(float a) {
+  const float y0 = __builtin_amdgcn_rcpf(a);
+  return fmaf(y0, fmaf(-a, y0, 1.0f), y0);
+}
 
 __device__ __forceinline__ double readlane_t(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -169,8 +183,9 @@ __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 +
 //     pivot columns are broadcast through a per-wave LDS buffer without
 //     barriers), lanes 16-63 hold 48 rows below it;
 //  2. the trailing lower tiles get A_IK -= L_IJ L_KJ^T (LDL: L_IJ D_J L_KJ^T)
-//     on MFMA 16x16x4, spread over waves 0-2, while wave 3 inverts the
-//     diagonal tile for the panel TRSM.
+//     on MFMA 16x16x4, four independent tiles per wave at a time.
+// Finally the 16x16 diagonal tiles are inverted (for the panel TRSM), all in
+// one pass with 16 lanes per tile.
 // Rows/columns beyond bs are padded with an identity block.  Task
 // {sn, step, inverse-tile offset, -}; info receives the smallest failing
 // pivot column (1-based, permuted order).
@@ -184,6 +199,7 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
   constexpr int NTL = NBM * (NBM + 1) / 2;
   __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
   __shared__ T dsh[SUB];
+  __shared__ T ysh[NBM * SUB];  // LL: 1/L_jj of every column
   __shared__ __attribute__((aligned(16))) T colb[NT / 64][SUB];
   const Task4 t = tasks[blockIdx.x];
   const SnInfo s = sn[t.a];
@@ -232,16 +248,17 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
     // every wave reads the diagonal tile, which wave 0 overwrites below
     __syncthreads();
-    T yv[SUB];  // 1/L_jj (LL) or 1/D_j (LDL)
+    T ylane = T(1);  // LL: 1/L_jj of this lane's diagonal row
     // column j of the diagonal rows (still unscaled) is broadcast through
     // this wave's LDS buffer: LDS operations of one wave complete in order,
     // so no barrier is needed between the write and the reads
     T* __restrict__ cb = colb[wave];
 #pragma unroll
     for (int j = 0; j < SUB; ++j) {
-      // the pivot and the next row's entry, which form the dependency chain
-      // from one pivot to the next, come by readlane; the rest of the
-      // column through LDS, off that chain
+      // The dependency chain from one pivot to the next runs through
+      // readlane(pivot) -> 1/pivot -> the next row's diagonal entry; the
+      // next row's entry comes by readlane, the rest of the column through
+      // LDS, and the scaling 1/sqrt(pivot) stays off the chain.
       if (diag_lane) cb[lane] = a[j];
       compiler_fence();
       T piv = readlane_t(a[j], j);
@@ -255,19 +272,18 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
         if (SUB * J + j < bs && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + SUB * J + j + 1);
         piv = T(1);
       }
-      if (LDL) {
-        const T inv = rcp_nr(piv);
-        yv[j] = inv;
-        const T l = a[j] * inv;
+      const T inv = rcp_cubic(piv);
+      // a_k -= (a_j / piv) u_k; the entry that feeds the next pivot is
+      // formed as (a_j u_{j+1}) * inv, one operation after inv
+      if (j + 1 < SUB) a[j + 1] = fma(-(a[j] * u[j + 1]), inv, a[j + 1]);
+      const T l = a[j] * inv;
 #pragma unroll
-        for (int k = j + 1; k < SUB; ++k) a[k] -= l * u[k];
+      for (int k = j + 2; k < SUB; ++k) a[k] -= l * u[k];
+      if (LDL) {
         a[j] = (lane == j) ? piv : l;
       } else {
         const T y = rsqrt_nr(piv);
-        yv[j] = y;
-        const T t2 = a[j] * y * y;
-#pragma unroll
-        for (int k = j + 1; k < SUB; ++k) a[k] -= t2 * u[k];
+        if (lane == j) ylane = y;
         a[j] = a[j] * y;
       }
     }
@@ -276,57 +292,77 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
 #pragma unroll
       for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (diag_lane && c > lane) ? T(0) : a[c];
     }
-    if (LDL && wave == 0 && diag_lane) dsh[lane] = a[lane];
+    if (wave == 0 && diag_lane) {
+      if (LDL) dsh[lane] = a[lane];
+      else ysh[SUB * J + lane] = ylane;
+    }
     __syncthreads();
-    if (wave == 3) {
-      // inverse of the factored diagonal tile (now in LDS), column `lane`
-      // per lane: forward substitution L z = e_lane (LDL: unit diagonal);
-      // overlaps the trailing update of waves 0-2
-      const T* Lt = tl + tslot(J, J) * TSZ;
+    // ---- 2. trailing update of tiles (I,K), J < K <= I < nb, on all waves:
+    // four independent tiles per wave at a time, their 16 MFMAs interleaved.
+    // Trailing tile q (row-major lower triangle) is (J+1+i, J+1+k) with
+    // q = i(i+1)/2 + k.
+    const int m = nb - J - 1;
+    const int ntr = m * (m + 1) / 2;
+    for (int q0 = 4 * wave; q0 < ntr; q0 += 16) {
+      v4 acc[4];
+      T av[4][4], bv[4][4];
+      int cofs[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int q = min(q0 + g, ntr - 1);
+        int ii = (int)((sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
+        while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
+        while (ii * (ii + 1) / 2 > q) --ii;
+        const int kk0 = q - ii * (ii + 1) / 2;
+        const int It = J + 1 + ii, Kt = J + 1 + kk0;
+        cofs[g] = tslot(It, Kt) * TSZ;
+        const T* A = tl + tslot(It, J) * TSZ;
+        const T* B = tl + tslot(Kt, J) * TSZ;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[g][v] = tl[cofs[g] + (lane & 15) * TSTR + Mfma<T>::row(lane, v)];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const int kp = Mfma<T>::row(lane, kk);
+          av[g][kk] = A[kp * TSTR + (lane & 15)];
+          if (LDL) av[g][kk] *= dsh[kp];
+          bv[g][kk] = B[kp * TSTR + (lane & 15)];
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) acc[g] = Mfma<T>::mma(-av[g][kk], bv[g][kk], acc[g]);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (q0 + g < ntr) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tl[cofs[g] + (lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[g][v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- inverses of the factored 16x16 diagonal tiles for the panel TRSM,
+  // all at once: 16 lanes per tile, lane (c) solves L z = e_c (LDL: unit
+  // diagonal) by forward substitution ----
+  {
+    const int Jt = 4 * wave + (lane >> 4);
+    const int c = lane & 15;
+    if (Jt < nb) {
+      const T* Lt = tl + tslot(Jt, Jt) * TSZ;
       T z[SUB];
 #pragma unroll
       for (int i = 0; i < SUB; ++i) {
-        T acc = (lane == i) ? T(1) : T(0);
+        T acc = (c == i) ? T(1) : T(0);
 #pragma unroll
         for (int k = 0; k < i; ++k) acc -= Lt[k * TSTR + i] * z[k];
-        z[i] = LDL ? acc : acc * yv[i];
+        z[i] = LDL ? acc : acc * ysh[SUB * Jt + i];
       }
-      if (diag_lane) {
-        T* out = dinv + (int64_t)(t.c + J) * (SUB * SUB) + lane * SUB;
+      T* out = dinv + (int64_t)(t.c + Jt) * (SUB * SUB) + c * SUB;
 #pragma unroll
-        for (int i = 0; i < SUB; ++i) out[i] = z[i];
-      }
+      for (int i = 0; i < SUB; ++i) out[i] = z[i];
     }
-    // ---- 2. trailing update of tiles (I,K), J < K <= I < nb ----
-    const int m = nb - J - 1;
-    const int ntr = m * (m + 1) / 2;
-    for (int q = wave; wave < 3 && q < ntr; q += 3) {
-      // decode q -> (K, I): columns K = J+1.., rows I = K..nb-1
-      int K = J + 1, rem = q, len = m;
-      while (rem >= len) {
-        rem -= len;
-        --len;
-        ++K;
-      }
-      const int It = K + rem;
-      T* C = tl + tslot(It, K) * TSZ;
-      const T* A = tl + tslot(It, J) * TSZ;
-      const T* B = tl + tslot(K, J) * TSZ;
-      v4 acc;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)];
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const int kp = Mfma<T>::row(lane, kk);
-        T av = A[kp * TSTR + (lane & 15)];
-        if (LDL) av *= dsh[kp];
-        const T bv = B[kp * TSTR + (lane & 15)];
-        acc = Mfma<T>::mma(-av, bv, acc);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[v];
-    }
-    __syncthreads();
   }
 
   // ---- write the factored lower triangle back ----
@@ -478,7 +514,9 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
                                                 T* __restrict__ arena,
                                                 const int64_t* __restrict__ tcol,
                                                 const int64_t* __restrict__ tmap,
-                                                const int32_t* __restrict__ rel, int64_t ntasks) {
+                                                const int32_t* __restrict__ rel,
+                                                const MergeInfo* __restrict__ merges,
+                                                const int32_t* __restrict__ mmap, int64_t ntasks) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int KC = 32;
   constexpr int LDS_LD = TILE + 1;  // odd pitch: the 4 k rows a wave reads hit different banks
@@ -526,17 +564,51 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   const bool brow_ok = brow0 + lrow < blim;
   const int arow = min(arow0 + lrow, alim - 1), brow = min(brow0 + lrow, blim - 1);
   constexpr int NQ = KC / 4;
+  // MODE 1 with t.d > 0: after the K columns of this panel, K2 more k
+  // columns from a merged child's panel, whose rows are gathered through
+  // the row map (rows the child lacks are zero)
+  int K2 = 0;
+  const T* __restrict__ P2 = P;
+  int64_t ld2 = 1;
+  int arow2 = 0, brow2 = 0;
+  bool arow2_ok = false, brow2_ok = false;
+  if (MODE == 1 && t.d > 0) {
+    const MergeInfo mg = merges[t.d - 1];
+    K2 = mg.w;
+    P2 = arena + mg.off;
+    ld2 = mg.ld;
+    const int ma = mmap[mg.moff + (arow - s.w)];
+    const int mb = mmap[mg.moff + (brow - s.w)];
+    arow2_ok = arow_ok && ma >= 0;
+    brow2_ok = brow_ok && mb >= 0;
+    arow2 = max(ma, 0);
+    brow2 = max(mb, 0);
+  }
+  const int nq1 = (K + KC - 1) / KC;
+  const int nq = nq1 + (K2 + KC - 1) / KC;
   T ra[NQ], rb[NQ];
-  auto fetch = [&](int kc) {
+  auto fetch = [&](int qc) {
+    if (qc < nq1) {
+      const int kc = qc * KC;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int kg = kc + lk + 4 * q;
-      const T* col = P + (int64_t)(kbeg + min(kg, K - 1)) * ld;
-      // clamped addresses hold finite values: masking by multiplication
-      // keeps the loads unconditional (a select would be turned into a
-      // predicated load with its own wait)
-      ra[q] = col[arow] * T((kg < K) & arow_ok);
-      rb[q] = col[brow] * T((kg < K) & brow_ok);
+      for (int q = 0; q < NQ; ++q) {
+        const int kg = kc + lk + 4 * q;
+        const T* col = P + (int64_t)(kbeg + min(kg, K - 1)) * ld;
+        // clamped addresses hold finite values: masking by multiplication
+        // keeps the loads unconditional (a select would be turned into a
+        // predicated load with its own wait)
+        ra[q] = col[arow] * T((kg < K) & arow_ok);
+        rb[q] = col[brow] * T((kg < K) & brow_ok);
+      }
+    } else {
+      const int kc = (qc - nq1) * KC;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int kg = kc + lk + 4 * q;
+        const T* col = P2 + (int64_t)min(kg, K2 - 1) * ld2;
+        ra[q] = col[arow2] * T((kg < K2) & arow2_ok);
+        rb[q] = col[brow2] * T((kg < K2) & brow2_ok);
+      }
     }
   };
   fetch(0);
@@ -569,18 +641,23 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         }
       }
   }
-  for (int kc = 0; kc < K; kc += KC) {
+  for (int qc = 0; qc < nq; ++qc) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       As[lk + 4 * q][lrow] = ra[q];
       Bs[lk + 4 * q][lrow] = rb[q];
     }
     if (LDL && tid < KC) {
-      const int kg = kc + tid;
-      Dk[tid] = (kg < K) ? P[(int64_t)(kbeg + kg) * ld + kbeg + kg] : T(0);
+      if (qc < nq1) {
+        const int kg = qc * KC + tid;
+        Dk[tid] = (kg < K) ? P[(int64_t)(kbeg + kg) * ld + kbeg + kg] : T(0);
+      } else {
+        const int kg = (qc - nq1) * KC + tid;
+        Dk[tid] = (kg < K2) ? P2[(int64_t)kg * ld2 + kg] : T(0);
+      }
     }
     __syncthreads();
-    if (kc + KC < K) fetch(kc + KC);  // next chunk in flight during the MFMAs
+    if (qc + 1 < nq) fetch(qc + 1);  // next chunk in flight during the MFMAs
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
       const int k = kk * 4 + (lane >> 4);
@@ -633,6 +710,8 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         }
   } else {
     const int m = s.r - s.w;
+    // a deferred supernode scatters only the output columns in its parent
+    const int bcut = t.d < 0 ? -t.d : m;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -641,7 +720,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 #pragma unroll
         for (int aj = 0; aj < 2; ++aj) {
           const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
-          if (boff < m && aoff < m && aoff >= boff) atomic_add(arena + off[bi][v][aj], -acc[bi][aj][v]);
+          if (boff < bcut && aoff < m && aoff >= boff) atomic_add(arena + off[bi][v][aj], -acc[bi][aj][v]);
         }
       }
   }

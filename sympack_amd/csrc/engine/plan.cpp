@@ -11,7 +11,7 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 }  // namespace
 
 void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
-                Plan& p) {
+                Plan& p, int merge_updates) {
   p = Plan();
   p.n = n;
   const Int ns = s.nsuper();
@@ -87,6 +87,91 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
         p.tcol[si.upd + bb] = sj.off + (int64_t)col * sj.r;
         p.tmap[si.upd + bb] = base - gb;
       }
+    }
+  }
+
+  // Deferred (merged) updates along supernode chains (see MergeInfo).  The
+  // parent P of C is the supernode of C's first off-diagonal row; the rows of
+  // struct(C) inside P's columns form a prefix of length n1, the remaining
+  // rows R2 all lie in struct(P).  P absorbs at most one child, and a
+  // supernode that absorbs a child is not itself deferred, so each merged
+  // update has exactly two k segments.  A child qualifies when R2 covers most
+  // of P's off-diagonal rows (the merged tiles then carry few zero rows).
+  // Merging moves work from the child's level to the parent's and lengthens
+  // the parent's tiles' k loop, which only pays where both levels have
+  // enough tiles to keep every CU busy (3 workgroups per CU x 256 CUs); near
+  // the root, where a level holds one or two supernodes, it would lengthen
+  // the critical path instead.
+  constexpr int32_t kMergeMinRows = 32;
+  constexpr double kMergeMinFill = 0.85;
+  const int64_t kMergeMinLevelTiles = merge_updates >= 2 ? 0 : 768;
+  std::vector<int64_t> level_tiles(p.nlevels, 0);
+  for (Int I = 0; I < ns; ++I) {
+    const int64_t t = ceil_div(p.sn[I].r - p.sn[I].w, kTile);
+    level_tiles[p.level[I]] += t * (t + 1) / 2;
+  }
+  p.deferred_into.assign(ns, -1);
+  std::vector<int32_t> merge_of(ns, 0);  // per supernode: 1 + index into merges, 0 = none
+  std::vector<int32_t> ncut(ns, 0);      // per deferred supernode: n1
+  if (merge_updates > 0) {
+    std::vector<int32_t> par(ns, -1), n1(ns, 0);
+    std::vector<std::vector<Int>> kids(ns);
+    for (Int I = 0; I < ns; ++I) {
+      const SnInfo& si = p.sn[I];
+      const int32_t m = si.r - si.w;
+      if (m <= 0) continue;
+      const int32_t* st = &p.lindx0[si.lx + si.w];
+      const Int P = row_sup[st[0]];
+      const int32_t end = p.sn[P].fc + p.sn[P].w;
+      int32_t k = 0;
+      while (k < m && st[k] < end) ++k;
+      par[I] = P;
+      n1[I] = k;
+      kids[P].push_back(I);
+    }
+    std::vector<char> is_merger(ns, 0);
+    std::vector<int32_t> map;
+    for (Int P = 0; P < ns; ++P) {  // children precede parents
+      const SnInfo& sp = p.sn[P];
+      const int32_t mP = sp.r - sp.w;
+      if (mP <= 0 || level_tiles[p.level[P]] < kMergeMinLevelTiles) continue;
+      Int best = -1;
+      int32_t bv = 0;
+      for (Int C : kids[P]) {
+        if (is_merger[C] || level_tiles[p.level[C]] < kMergeMinLevelTiles) continue;
+        const int32_t r2 = (p.sn[C].r - p.sn[C].w) - n1[C];
+        if (r2 >= kMergeMinRows && (double)r2 >= kMergeMinFill * mP && r2 > bv) {
+          best = C;
+          bv = r2;
+        }
+      }
+      if (best < 0) continue;
+      const SnInfo& sc = p.sn[best];
+      const int32_t* stC = &p.lindx0[sc.lx];
+      for (int32_t q = 0; q < sc.r; ++q) pos[stC[q]] = q;
+      map.assign(mP, -1);
+      int32_t found = 0;
+      for (int32_t a = 0; a < mP; ++a) {
+        const int32_t q = pos[p.lindx0[sp.lx + sp.w + a]];
+        if (q >= sc.w) {
+          map[a] = q;
+          ++found;
+        }
+      }
+      for (int32_t q = 0; q < sc.r; ++q) pos[stC[q]] = -1;
+      if (found != bv) continue;  // R2 not inside struct(P): keep the plain update
+      MergeInfo mi{};
+      mi.off = sc.off;
+      mi.moff = (int64_t)p.mmap.size();
+      mi.ld = sc.r;
+      mi.w = sc.w;
+      mi.child = best;
+      p.mmap.insert(p.mmap.end(), map.begin(), map.end());
+      is_merger[P] = 1;
+      p.deferred_into[best] = P;
+      ncut[best] = n1[best];
+      merge_of[P] = (int32_t)p.merges.size() + 1;
+      p.merges.push_back(mi);
     }
   }
 
@@ -181,10 +266,24 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       int64_t m = si.r - si.w;
       if (m <= 0) continue;
       int64_t nt = ceil_div(m, kTile);
-      for (int64_t ti = 0; ti < nt; ++ti)
-        for (int64_t tj = 0; tj <= ti; ++tj) p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, 0});
+      if (p.deferred_into[I] >= 0) {
+        // only the output columns in the parent's columns; the rest is
+        // computed by the parent's merged update
+        const int32_t n1 = ncut[I];
+        const int64_t ntj = ceil_div(n1, kTile);
+        for (int64_t ti = 0; ti < nt; ++ti)
+          for (int64_t tj = 0; tj <= ti && tj < ntj; ++tj)
+            p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, -n1});
+        p.schur_entries += (int64_t)n1 * m - (int64_t)n1 * (n1 - 1) / 2;
+      } else {
+        for (int64_t ti = 0; ti < nt; ++ti)
+          for (int64_t tj = 0; tj <= ti; ++tj)
+            p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, merge_of[I]});
+        p.schur_entries += m * (m + 1) / 2;
+      }
+      // algorithmic flops stay those of the supernode's whole update, wherever
+      // its deferred part is computed
       p.schur_flops += (double)si.w * (double)m * (double)(m + 1);
-      p.schur_entries += m * (m + 1) / 2;
     }
     sr.count = (int64_t)p.schur_tasks.size() - sr.first;
     if (sr.count) p.factor_seq.push_back(sr);

@@ -42,6 +42,21 @@ struct Task4 {
   int32_t a, b, c, d;
 };
 
+// A child supernode C whose Schur update is split in two: the part landing in
+// its parent P's columns is applied at C's level; the rest (rows and columns
+// both in struct(P)) is deferred and computed inside P's Schur update as
+// extra k columns, so those target entries are scattered once instead of
+// twice.  mmap[moff + a] = row of C's panel holding P's off-diagonal row a
+// (-1 where C has no such row).
+struct MergeInfo {         // mirrored on the device
+  int64_t off;             // C's panel offset
+  int64_t moff;            // offset of the row map in mmap
+  int32_t ld;              // C's panel leading dimension (rows)
+  int32_t w;               // C's width (extra k columns)
+  int32_t child;           // C
+  int32_t pad;
+};
+
 struct LaunchRange {
   int kind;          // 0 diagonal factor, 7 panel TRSM, 1 in-panel update, 2 Schur update,
                      // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
@@ -72,7 +87,13 @@ struct Plan {
   int64_t inv_tiles = 0;           // 16x16 inverse diagonal tiles written by the factor
   std::vector<int64_t> inv_base;   // per supernode: first inverse tile (tile t of 16-column block t)
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
-  std::vector<Task4> schur_tasks;  // {sn, ti, tj, 0}: 64x64 output tiles
+  // {sn, ti, tj, d}: 64x64 output tiles; d = 0 plain, d > 0 the update also
+  // carries merges[d-1] (a deferred child), d < 0 only output columns b < -d
+  // (parent columns of a deferred supernode)
+  std::vector<Task4> schur_tasks;
+  std::vector<MergeInfo> merges;
+  std::vector<int32_t> mmap;
+  std::vector<int32_t> deferred_into;  // per supernode: parent absorbing its update, -1 = none
   std::vector<LaunchRange> factor_seq;
   // solve
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
@@ -85,8 +106,10 @@ struct Plan {
 };
 
 // Build the plan from the symbolic analysis and the caller's CSC pattern
-// (lower triangle, any base).
+// (lower triangle, any base).  merge_updates: 0 no merged updates, 1 merge
+// qualifying chain children on levels with enough tiles to fill the GPU,
+// 2 merge every qualifying chain child.
 void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
-                Plan& p);
+                Plan& p, int merge_updates = 1);
 
 }  // namespace spk

@@ -50,7 +50,7 @@ class symPACKOptions:
 
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
-                 profile=False, device=0):
+                 profile=False, device=0, merge_updates=1):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -61,6 +61,7 @@ class symPACKOptions:
         self.verbose = verbose
         self.profile = profile
         self.device = device
+        self.merge_updates = merge_updates
 
     @classmethod
     def from_args(cls, args):
@@ -75,7 +76,8 @@ class symPACKOptions:
                                  o.zrelax0, o.zrelax1, o.zrelax2)
         return cls(ordering=inv[o.ordering], decomposition="LDL" if o.decomposition else "LL",
                    relax=r, precision="FP32_IR" if o.precision else "FP64",
-                   ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose)
+                   ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
+                   merge_updates=int(o.merge_updates))
 
     def to_c(self):
         lib = _lib.load()
@@ -96,6 +98,7 @@ class symPACKOptions:
         o.verbose = self.verbose
         o.profile = 1 if self.profile else 0
         o.device = self.device
+        o.merge_updates = int(self.merge_updates)
         return o
 
 
@@ -321,6 +324,13 @@ class symPACKMatrix2D:
         _lib.check(self._lib.spk_get_structure(self._h, _ptr(xl, ctypes.c_int64),
                                                _ptr(lx, ctypes.c_int32)), "lindx")
         return xl, lx
+
+    def GetUpdateMerges(self):
+        """deferred_into[I] (0-based): the parent that computes supernode I's
+        update outside the parent's columns, -1 where I scatters all of it."""
+        d = np.zeros(self.TotalSupernodeCnt(), np.int32)
+        _lib.check(self._lib.spk_get_update_merges(self._h, _ptr(d, ctypes.c_int32)), "merges")
+        return d
 
     def GetCells(self):
         nc = ctypes.c_int64()
