@@ -169,6 +169,23 @@ __device__ __forceinline__ float readlane_t(float v, int lane) {
 // ---------------------------------------------------------------------------
 constexpr int SUB = kSub;
 constexpr int TSTR = SUB + 1;
+// Trailing tiles (I, K) = (i+1, k+1), 0 <= k <= i <= nb-2, of a diagonal
+// block with nb 16-column blocks, ordered by k descending (then i):
+// kOrd[kOrdOff[nb] + t] = (i << 4) | k.  The tiles still to be updated at
+// block step J (k >= J) are then a prefix of the list.
+__constant__ unsigned char kOrdOff[12] = {0, 0, 0, 1, 4, 10, 20, 35, 56, 84, 120, 165};
+__constant__ unsigned char kOrd[165] = {
+    0x00, 0x11, 0x00, 0x10, 0x22, 0x11, 0x21, 0x00, 0x10, 0x20, 0x33, 0x22, 0x32, 0x11, 0x21,
+    0x31, 0x00, 0x10, 0x20, 0x30, 0x44, 0x33, 0x43, 0x22, 0x32, 0x42, 0x11, 0x21, 0x31, 0x41,
+    0x00, 0x10, 0x20, 0x30, 0x40, 0x55, 0x44, 0x54, 0x33, 0x43, 0x53, 0x22, 0x32, 0x42, 0x52,
+    0x11, 0x21, 0x31, 0x41, 0x51, 0x00, 0x10, 0x20, 0x30, 0x40, 0x50, 0x66, 0x55, 0x65, 0x44,
+    0x54, 0x64, 0x33, 0x43, 0x53, 0x63, 0x22, 0x32, 0x42, 0x52, 0x62, 0x11, 0x21, 0x31, 0x41,
+    0x51, 0x61, 0x00, 0x10, 0x20, 0x30, 0x40, 0x50, 0x60, 0x77, 0x66, 0x76, 0x55, 0x65, 0x75,
+    0x44, 0x54, 0x64, 0x74, 0x33, 0x43, 0x53, 0x63, 0x73, 0x22, 0x32, 0x42, 0x52, 0x62, 0x72,
+    0x11, 0x21, 0x31, 0x41, 0x51, 0x61, 0x71, 0x00, 0x10, 0x20, 0x30, 0x40, 0x50, 0x60, 0x70,
+    0x88, 0x77, 0x87, 0x66, 0x76, 0x86, 0x55, 0x65, 0x75, 0x85, 0x44, 0x54, 0x64, 0x74, 0x84,
+    0x33, 0x43, 0x53, 0x63, 0x73, 0x83, 0x22, 0x32, 0x42, 0x52, 0x62, 0x72, 0x82, 0x11, 0x21,
+    0x31, 0x41, 0x51, 0x61, 0x71, 0x81, 0x00, 0x10, 0x20, 0x30, 0x40, 0x50, 0x60, 0x70, 0x80};
 constexpr int TSZ = SUB * TSTR;
 __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 + J; }
 
@@ -208,7 +225,8 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
   const int nb = (bs + SUB - 1) / SUB;
   const int64_t ld = s.r;
   T* __restrict__ P = arena + s.off + (int64_t)c0 * ld + c0;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar control
 
   // ---- load the lower tiles (clamped unconditional loads, one wait) ----
   {
@@ -234,9 +252,40 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
   }
   __syncthreads();
 
+  // Trailing tiles live in MFMA accumulators for the whole factorization:
+  // tile t of the k-descending order belongs to wave t % 4, slot t / 4, and
+  // goes back to LDS when its block column becomes the panel.
+  constexpr int NS = ((NBM - 1) * NBM / 2 + 3) / 4;
+  const int ntt = (nb - 1) * nb / 2;
+  const unsigned char* __restrict__ ord = kOrd + kOrdOff[nb];
+  v4 acc[NS > 0 ? NS : 1];
+  int tI[NS > 0 ? NS : 1], tK[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const int t = 4 * q + wave;
+    const int e = ord[min(t, max(ntt - 1, 0))];
+    tI[q] = (t < ntt) ? (e >> 4) + 1 : 0;
+    tK[q] = (t < ntt) ? (e & 15) + 1 : NBM;  // NBM: never active
+    const T* C = tl + tslot(max(tI[q], tK[q] < NBM ? tK[q] : 0), tK[q] < NBM ? tK[q] : 0) * TSZ;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[q][v] = C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)];
+  }
+
   const bool diag_lane = lane < SUB;
   const int rb = 48 * wave + lane - SUB;  // row below the diagonal tile (lanes >= 16)
   for (int J = 0; J < nb; ++J) {
+    // ---- 0. the tiles of block column J leave the accumulators ----
+    if (J > 0) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        if (tK[q] == J) {
+          T* C = tl + tslot(tI[q], J) * TSZ;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[q][v];
+        }
+      }
+      __syncthreads();
+    }
     // ---- 1. block column J, lane per row ----
     const int Ib = J + 1 + (diag_lane ? 0 : rb / SUB);
     const bool row_ok = diag_lane || (Ib < nb);
@@ -249,6 +298,8 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     // every wave reads the diagonal tile, which wave 0 overwrites below
     __syncthreads();
     T ylane = T(1);  // LL: 1/L_jj of this lane's diagonal row
+    T dlane = T(1);  // pivot d_j of diagonal row j (lane j)
+    int first_bad = INT_MAX;  // first failing pivot column of this block column (uniform)
     // column j of the diagonal rows (still unscaled) is broadcast through
     // this wave's LDS buffer: LDS operations of one wave complete in order,
     // so no barrier is needed between the write and the reads
@@ -261,29 +312,37 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
       // LDS, and the scaling 1/sqrt(pivot) stays off the chain.
       if (diag_lane) cb[lane] = a[j];
       compiler_fence();
-      T piv = readlane_t(a[j], j);
+      const T piv = readlane_t(a[j], j);
       T u[SUB];
+      // u[j+1] and u[j+2] feed the next two pivots: readlane; the rest
+      // through LDS, whose latency then has two steps of slack
       if (j + 1 < SUB) u[j + 1] = readlane_t(a[j], j + 1);
+      if (j + 2 < SUB) u[j + 2] = readlane_t(a[j], j + 2);
 #pragma unroll
-      for (int k = j + 2; k < SUB; ++k) u[k] = cb[k];
+      for (int k = j + 3; k < SUB; ++k) u[k] = cb[k];
       compiler_fence();
+      // a failing pivot is recorded off the chain; the factor is then
+      // invalid anyway, the values that follow are not used
       const bool bad = LDL ? (piv == T(0) || piv != piv) : !(piv > T(0));
-      if (bad) {
-        if (SUB * J + j < bs && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + SUB * J + j + 1);
-        piv = T(1);
-      }
+      if (bad && SUB * J + j < bs) first_bad = min(first_bad, SUB * J + j);
       const T inv = rcp_cubic(piv);
+      if (lane == j) dlane = piv;
       // a_k -= (a_j / piv) u_k; the entry that feeds the next pivot is
       // formed as (a_j u_{j+1}) * inv, one operation after inv
       if (j + 1 < SUB) a[j + 1] = fma(-(a[j] * u[j + 1]), inv, a[j + 1]);
       const T l = a[j] * inv;
 #pragma unroll
       for (int k = j + 2; k < SUB; ++k) a[k] -= l * u[k];
-      if (LDL) {
-        a[j] = (lane == j) ? piv : l;
-      } else {
-        const T y = rsqrt_nr(piv);
-        if (lane == j) ylane = y;
+      // LL keeps the unscaled column (a_ij) for now; LDL stores l = a_ij / d_j
+      if (LDL) a[j] = (lane == j) ? piv : l;
+    }
+    if (!LDL) {
+      // L_ij = a_ij / sqrt(d_j): lane j computes 1/sqrt(d_j) for its
+      // diagonal row, the 16 factors are broadcast by readlane
+      ylane = rsqrt_nr(dlane);
+#pragma unroll
+      for (int j = 0; j < SUB; ++j) {
+        const T y = readlane_t(ylane, j);
         a[j] = a[j] * y;
       }
     }
@@ -296,48 +355,38 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
       if (LDL) dsh[lane] = a[lane];
       else ysh[SUB * J + lane] = ylane;
     }
+    if (first_bad != INT_MAX && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + first_bad + 1);
     __syncthreads();
-    // ---- 2. trailing update of tiles (I,K), J < K <= I < nb, on all waves:
-    // four independent tiles per wave at a time, their 16 MFMAs interleaved.
-    // Trailing tile q (row-major lower triangle) is (J+1+i, J+1+k) with
-    // q = i(i+1)/2 + k.
+    // ---- 2. trailing update A_IK -= L_IJ L_KJ^T of the register tiles with
+    // K > J (a prefix of each wave's slots); operands of four slots are
+    // loaded together, the MFMAs of the active ones issue back to back ----
     const int m = nb - J - 1;
     const int ntr = m * (m + 1) / 2;
-    for (int q0 = 4 * wave; q0 < ntr; q0 += 16) {
-      v4 acc[4];
-      T av[4][4], bv[4][4];
-      int cofs[4];
+    const int na = (ntr - wave + 3) >> 2;  // active slots of this wave
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int q = min(q0 + g, ntr - 1);
-        int ii = (int)((sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
-        while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-        while (ii * (ii + 1) / 2 > q) --ii;
-        const int kk0 = q - ii * (ii + 1) / 2;
-        const int It = J + 1 + ii, Kt = J + 1 + kk0;
-        cofs[g] = tslot(It, Kt) * TSZ;
-        const T* A = tl + tslot(It, J) * TSZ;
-        const T* B = tl + tslot(Kt, J) * TSZ;
+    for (int g = 0; g < NS; g += 4) {
+      if (g < na) {
+        T av[4][4], bv[4][4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[g][v] = tl[cofs[g] + (lane & 15) * TSTR + Mfma<T>::row(lane, v)];
+        for (int h = 0; h < 4; ++h) {
+          const int q = min(g + h, NS - 1);
+          const bool on = g + h < na;
+          const T* A = tl + tslot(on ? tI[q] : J, J) * TSZ;
+          const T* B = tl + tslot(on ? tK[q] : J, J) * TSZ;
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          const int kp = Mfma<T>::row(lane, kk);
-          av[g][kk] = A[kp * TSTR + (lane & 15)];
-          if (LDL) av[g][kk] *= dsh[kp];
-          bv[g][kk] = B[kp * TSTR + (lane & 15)];
+          for (int kk = 0; kk < 4; ++kk) {
+            const int kp = Mfma<T>::row(lane, kk);
+            av[h][kk] = A[kp * TSTR + (lane & 15)] * (on ? T(1) : T(0));
+            if (LDL) av[h][kk] *= dsh[kp];
+            bv[h][kk] = B[kp * TSTR + (lane & 15)];
+          }
         }
-      }
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
+        for (int h = 0; h < 4; ++h)
+          if (g + h < NS && g + h < na) {
 #pragma unroll
-        for (int g = 0; g < 4; ++g) acc[g] = Mfma<T>::mma(-av[g][kk], bv[g][kk], acc[g]);
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        if (q0 + g < ntr) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) tl[cofs[g] + (lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[g][v];
-        }
+            for (int kk = 0; kk < 4; ++kk) acc[g + h] = Mfma<T>::mma(-av[h][kk], bv[h][kk], acc[g + h]);
+          }
       }
     }
     __syncthreads();
@@ -365,16 +414,20 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
     }
   }
 
-  // ---- write the factored lower triangle back ----
+  // ---- write the factored lower triangle back (LDS reads first, one wait) ----
   {
     const int r = tid & 15, c = tid >> 4;
+    T v[NTL];
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) v[q] = tl[q * TSZ + c * TSTR + r];
+#pragma unroll
+    for (int q = 0; q < NTL; ++q) keep(v[q]);
 #pragma unroll
     for (int I = 0; I < NBM; ++I)
 #pragma unroll
       for (int J = 0; J <= I; ++J) {
         const int row = SUB * I + r, col = SUB * J + c;
-        if (row < bs && col < bs && row >= col)
-          P[(int64_t)col * ld + row] = tl[tslot(I, J) * TSZ + c * TSTR + r];
+        if (row < bs && col < bs && row >= col) P[(int64_t)col * ld + row] = v[tslot(I, J)];
       }
   }
 }
