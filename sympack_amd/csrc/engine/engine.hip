@@ -564,6 +564,18 @@ template <typename T, bool BWD>
 void Engine::launch_sdiag(int bucket, unsigned g, const Task4* t, const T* arena, const T* Y, T* X, int nrhs) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
   const T* dinv = (const T*)d_dinv_;
+  if (nrhs <= 16) {
+    // one right-hand-side group: blocks spread over the waves
+    switch (bucket) {
+      case 1: dev::k_sdiag4<T, BWD, 1><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+      case 2: dev::k_sdiag4<T, BWD, 2><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+      case 4: dev::k_sdiag4<T, BWD, 4><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+      case 6: dev::k_sdiag4<T, BWD, 6><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+      case 10: dev::k_sdiag4<T, BWD, 10><<<g, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;
+      default: throw std::logic_error("bad diagonal-block bucket");
+    }
+    return;
+  }
   const dim3 grid(g, (unsigned)((nrhs + 63) / 64));
   switch (bucket) {
     case 1: dev::k_sdiag3<T, BWD, 1><<<grid, 256, 0, stream_>>>(sn, t, arena, dinv, Y, X, nrhs); break;

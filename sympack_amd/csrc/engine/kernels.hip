@@ -991,6 +991,114 @@ __global__ __launch_bounds__(NT) void k_sdiag3(const SnInfo* __restrict__ sn,
   }
 }
 
+// Diagonal-block solve for up to 16 right-hand sides (w <= 160), the
+// 16-row blocks spread over the four waves (block K belongs to wave K % 4,
+// its x^T in MFMA accumulators).  Each wave loads the tiles it needs (the
+// L tiles that fold other blocks into its own, and its inverse diagonal
+// tiles) straight from HBM into MFMA operand registers, all before the
+// first product.  When block K is final its owner publishes it in LDS and
+// every wave folds it into the blocks it owns that still wait for it
+// (forward: K' > K; backward: K' < K), the next block first.  Same
+// recurrence as k_sdiag3, which handles each right-hand-side group with one
+// wave.  Task {sn, inverse-tile base, -, -}; Y may alias X.
+template <typename T, bool BWD, int NBM>
+__global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
+                                               const Task4* __restrict__ tasks,
+                                               const T* __restrict__ arena, const T* __restrict__ dinv,
+                                               const T* Y, T* X, int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NO = (NBM + 3) / 4;  // blocks per wave
+  __shared__ __attribute__((aligned(16))) T xs[NBM][4][64];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int w = s.w;
+  const int nbt = (w + SUB - 1) / SUB;
+  const T* __restrict__ P = arena + s.off;
+  const int64_t ld = s.r;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const T* __restrict__ Dv = dinv + (int64_t)t.b * (SUB * SUB);
+  const int m16 = lane & 15;
+  // A operands: Lf[i][K][kk] folds block K into owned block i; Li[i][kk] is
+  // the inverse diagonal tile of owned block i.  L(r, c) sits at
+  // P[c ld + r] (clamped inside the block).
+  T Lf[NO][NBM][4], Li[NO][4];
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    const int K2 = wave + 4 * i;
+#pragma unroll
+    for (int K = 0; K < NBM; ++K)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        // forward: L(16 K2 + m16, 16 K + kp); backward (L^T): L(16 K + kp, 16 K2 + m16)
+        const int rr = BWD ? SUB * K + kp : SUB * K2 + m16;
+        const int cc = BWD ? SUB * K2 + m16 : SUB * K + kp;
+        // zero outside the block: the padding entries of x are not zero
+        const bool need = K2 < nbt && K < nbt && (BWD ? K > K2 : K < K2) && rr < w && cc < w;
+        Lf[i][K][kk] = need ? P[(int64_t)min(cc, w - 1) * ld + min(rr, w - 1)] : T(0);
+      }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int kp = Mfma<T>::row(lane, kk);
+      const int Kc = min(K2, nbt - 1);
+      // forward uses inv(L_KK)(m16, kp), backward its transpose
+      Li[i][kk] = Dv[(int64_t)Kc * (SUB * SUB) + (BWD ? m16 * SUB + kp : kp * SUB + m16)];
+    }
+  }
+  const int j = lane & 15;
+  const int jcl = min(j, nrhs - 1);
+  const T jm = T(j < nrhs);
+  const int64_t base = (int64_t)s.fc * nrhs + jcl;
+  v4 xo[NO];
+#pragma unroll
+  for (int i = 0; i < NO; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = min(SUB * (wave + 4 * i) + Mfma<T>::row(lane, v), w - 1);
+      xo[i][v] = Y[base + (int64_t)row * nrhs] * jm;
+    }
+#pragma unroll
+  for (int Ks = 0; Ks < NBM; ++Ks) {
+    const int K = BWD ? NBM - 1 - Ks : Ks;
+    if (K < nbt) {
+      if ((K & 3) == wave) {
+        // block K is complete: apply the inverse diagonal tile, publish
+        const int i = K >> 2;
+        v4 z = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) z = Mfma<T>::mma(Li[i][kk], xo[i][kk], z);
+        xo[i] = z;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xs[K][v][lane] = z[v];
+      }
+      __syncthreads();
+      // fold block K into this wave's waiting blocks, the next one first
+#pragma unroll
+      for (int ii = 0; ii < NO; ++ii) {
+        const int i = BWD ? NO - 1 - ii : ii;
+        const int K2 = wave + 4 * i;
+        if (K2 < nbt && (BWD ? K2 < K : K2 > K)) {
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) xo[i] = Mfma<T>::mma(-Lf[i][K][kk], xs[K][kk][lane], xo[i]);
+        }
+      }
+    }
+  }
+  if (j >= nrhs) return;
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    const int K = wave + 4 * i;
+    if (K < nbt) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = SUB * K + Mfma<T>::row(lane, v);
+        if (row < w) X[base + (int64_t)row * nrhs] = xo[i][v];
+      }
+    }
+  }
+}
+
 // Multi-RHS off-diagonal solve updates on MFMA (used for nrhs >= 4), one
 // workgroup per (64 panel rows below the diagonal block, 16 right-hand
 // sides); wave q owns rows 16q..16q+15.  Task {sn, row tile, -, -},
@@ -1065,32 +1173,46 @@ __global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
   const T jmask = T(jn < nrhs);
   const int64_t ld = s.r;
   // B operand (shared by every column block): Y rows of this tile
+  int rowi[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) rowi[st] = lindx0[s.lx + s.w + min(a0 + 16 * g + st, m - 1)];
   T bv[16];
 #pragma unroll
   for (int st = 0; st < 16; ++st) {
     const int a = a0 + 16 * g + st;
-    const int acl = min(a, m - 1);
-    const int row = lindx0[s.lx + s.w + acl];
-    bv[st] = X[(int64_t)row * nrhs + jc] * (jmask * T(a < m));
+    bv[st] = X[(int64_t)rowi[st] * nrhs + jc] * (jmask * T(a < m));
   }
+  // the column blocks of this wave (up to three per pass, w <= 192): every
+  // load issued before the first MFMA
   const int nkb = (s.w + 15) >> 4;
-  for (int kb = wave; kb < nkb; kb += NT / 64) {
-    const int kcol = min(kb * 16 + mi, s.w - 1);
-    const T* __restrict__ Pc = arena + s.off + (int64_t)kcol * ld + s.w;
-    T av[16];
+  constexpr int MKB = 3;
+  for (int kb0 = wave; kb0 < nkb; kb0 += MKB * (NT / 64)) {
+    T av[MKB][16];
 #pragma unroll
-    for (int st = 0; st < 16; ++st) {
-      const int a = min(a0 + 16 * g + st, m - 1);
-      av[st] = Pc[a];
+    for (int i = 0; i < MKB; ++i) {
+      const int kcol = min((kb0 + i * (NT / 64)) * 16 + mi, s.w - 1);
+      const T* __restrict__ Pc = arena + s.off + (int64_t)kcol * ld + s.w;
+#pragma unroll
+      for (int st = 0; st < 16; ++st) av[i][st] = Pc[min(a0 + 16 * g + st, m - 1)];
     }
-    v4 acc = v4{0, 0, 0, 0};
 #pragma unroll
-    for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[st], bv[st], acc);
-    if (jn < nrhs) {
+    for (int i = 0; i < MKB; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int k = kb * 16 + Mfma<T>::row(lane, v);
-        if (k < s.w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, -acc[v]);
+      for (int st = 0; st < 16; ++st) keep(av[i][st]);
+#pragma unroll
+    for (int i = 0; i < MKB; ++i) {
+      const int kb = kb0 + i * (NT / 64);
+      if (kb < nkb) {
+        v4 acc = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[i][st], bv[st], acc);
+        if (jn < nrhs) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int k = kb * 16 + Mfma<T>::row(lane, v);
+            if (k < s.w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, -acc[v]);
+          }
+        }
       }
     }
   }
