@@ -1019,33 +1019,9 @@ __global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const T* __restrict__ Dv = dinv + (int64_t)t.b * (SUB * SUB);
   const int m16 = lane & 15;
-  // A operands: Lf[i][K][kk] folds block K into owned block i; Li[i][kk] is
-  // the inverse diagonal tile of owned block i.  L(r, c) sits at
-  // P[c ld + r] (clamped inside the block).
-  T Lf[NO][NBM][4], Li[NO][4];
-#pragma unroll
-  for (int i = 0; i < NO; ++i) {
-    const int K2 = wave + 4 * i;
-#pragma unroll
-    for (int K = 0; K < NBM; ++K)
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const int kp = Mfma<T>::row(lane, kk);
-        // forward: L(16 K2 + m16, 16 K + kp); backward (L^T): L(16 K + kp, 16 K2 + m16)
-        const int rr = BWD ? SUB * K + kp : SUB * K2 + m16;
-        const int cc = BWD ? SUB * K2 + m16 : SUB * K + kp;
-        // zero outside the block: the padding entries of x are not zero
-        const bool need = K2 < nbt && K < nbt && (BWD ? K > K2 : K < K2) && rr < w && cc < w;
-        Lf[i][K][kk] = need ? P[(int64_t)min(cc, w - 1) * ld + min(rr, w - 1)] : T(0);
-      }
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const int kp = Mfma<T>::row(lane, kk);
-      const int Kc = min(K2, nbt - 1);
-      // forward uses inv(L_KK)(m16, kp), backward its transpose
-      Li[i][kk] = Dv[(int64_t)Kc * (SUB * SUB) + (BWD ? m16 * SUB + kp : kp * SUB + m16)];
-    }
-  }
+  // Loads in the order the recurrence consumes them (right-hand sides,
+  // inverse tiles, then the fold operands block by block), so the first
+  // steps wait only for the first loads.
   const int j = lane & 15;
   const int jcl = min(j, nrhs - 1);
   const T jm = T(j < nrhs);
@@ -1058,6 +1034,36 @@ __global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
       const int row = min(SUB * (wave + 4 * i) + Mfma<T>::row(lane, v), w - 1);
       xo[i][v] = Y[base + (int64_t)row * nrhs] * jm;
     }
+  // Li[i][kk]: inverse diagonal tile of owned block i; Lf[i][K][kk] folds
+  // block K into owned block i.  L(r, c) sits at P[c ld + r].
+  T Lf[NO][NBM][4], Li[NO][4];
+#pragma unroll
+  for (int i = 0; i < NO; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int kp = Mfma<T>::row(lane, kk);
+      const int Kc = min(wave + 4 * i, nbt - 1);
+      // forward uses inv(L_KK)(m16, kp), backward its transpose
+      Li[i][kk] = Dv[(int64_t)Kc * (SUB * SUB) + (BWD ? m16 * SUB + kp : kp * SUB + m16)];
+    }
+#pragma unroll
+  for (int Ks = 0; Ks < NBM; ++Ks) {
+    const int K = BWD ? NBM - 1 - Ks : Ks;
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      const int K2 = wave + 4 * i;
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        // forward: L(16 K2 + m16, 16 K + kp); backward (L^T): L(16 K + kp, 16 K2 + m16)
+        const int rr = BWD ? SUB * K + kp : SUB * K2 + m16;
+        const int cc = BWD ? SUB * K2 + m16 : SUB * K + kp;
+        // zero outside the block: the padding entries of x are not zero
+        const bool need = K2 < nbt && K < nbt && (BWD ? K > K2 : K < K2) && rr < w && cc < w;
+        Lf[i][K][kk] = need ? P[(int64_t)min(cc, w - 1) * ld + min(rr, w - 1)] : T(0);
+      }
+    }
+  }
 #pragma unroll
   for (int Ks = 0; Ks < NBM; ++Ks) {
     const int K = BWD ? NBM - 1 - Ks : Ks;
