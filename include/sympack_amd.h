@@ -66,6 +66,9 @@ typedef struct spk_options {
                          computed inside the parent's Schur update (one scatter instead of two)
                          on etree levels with enough update tiles to fill the GPU; 2: on every
                          level; 0: every supernode scatters its whole update */
+  int solve_dataflow; /* must be 0 (default): one launch per etree level and kernel kind.
+                         The dependency-driven sweeps (one persistent launch each) are
+                         disabled in this build; Solve rejects 1 */
 } spk_options;
 
 typedef struct spk_stats {
@@ -163,6 +166,10 @@ int spk_get_cells(spk_handle h, int32_t* cell_K, int32_t* cell_I, int32_t* cell_
  * deferred_into[I] = 0-based parent that computes the part of supernode I's
  * update outside the parent's columns, -1 where I scatters its whole update. */
 int spk_get_update_merges(spk_handle h, int32_t* deferred_into /* nsuper */);
+/* Checks the task lists of the dependency-driven solve (no counterpart in
+ * the reference): 0 when a single worker taking tasks in ticket order never
+ * waits and every counter drains, < 0 with spk_last_error() otherwise. */
+int spk_validate_solve_plan(spk_handle h);
 /* Factor panel of supernode I (1-based): r x w column-major, structure rows
  * by supernode columns; LDL panels hold D on the diagonal. */
 int spk_get_panel_dims(spk_handle h, int32_t I, int32_t* nrows, int32_t* ncols);

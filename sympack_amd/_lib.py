@@ -34,6 +34,7 @@ class SpkOptions(ctypes.Structure):
         ("profile", ctypes.c_int),
         ("device", ctypes.c_int),
         ("merge_updates", ctypes.c_int),
+        ("solve_dataflow", ctypes.c_int),
     ]
 
 
@@ -104,6 +105,7 @@ SIGNATURES = [
     ("spk_get_structure", _I, [_P, _PI64, _PI32]),
     ("spk_get_cell_counts", _I, [_P, _PI64, _PI64]),
     ("spk_get_update_merges", _I, [_P, _PI32]),
+    ("spk_validate_solve_plan", _I, [_P]),
     ("spk_get_cells", _I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32, _PI32]),
     ("spk_get_panel_dims", _I, [_P, _I32, _PI32, _PI32]),
     ("spk_get_factor_panel", _I, [_P, _I32, _PD]),

@@ -73,6 +73,7 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   e.profile = o.profile;
   e.device = o.device;
   e.merge_updates = o.merge_updates;
+  e.solve_dataflow = o.solve_dataflow;
   return e;
 }
 }  // namespace
@@ -128,6 +129,7 @@ void spk_default_options(spk_options* o) {
   o->profile = 0;
   o->device = g_device;
   o->merge_updates = 1;
+  o->solve_dataflow = 0;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -185,6 +187,8 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         o->ir_max_iter = std::atoi(val(1).c_str());
       } else if (k == "-merge") {
         o->merge_updates = std::atoi(val(1).c_str());
+      } else if (k == "-dataflow") {
+        o->solve_dataflow = std::atoi(val(1).c_str()) != 0;
       }
     }
     return 0;
@@ -430,6 +434,15 @@ int spk_get_structure(spk_handle h, int64_t* xlindx, int32_t* lindx) {
   if (xlindx) std::memcpy(xlindx, s.xlindx.data(), s.xlindx.size() * sizeof(int64_t));
   if (lindx) std::memcpy(lindx, s.lindx.data(), s.lindx.size() * sizeof(int32_t));
   return 0;
+}
+
+int spk_validate_solve_plan(spk_handle h) {
+  if (!h) return fail("null handle");
+  return guarded([&]() {
+    const std::string e = spk::validate_solve_plan(h->engine.plan());
+    if (!e.empty()) throw std::logic_error(e);
+    return 0;
+  });
 }
 
 int spk_get_update_merges(spk_handle h, int32_t* deferred_into) {

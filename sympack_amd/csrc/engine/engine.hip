@@ -76,6 +76,15 @@ void Engine::free_device() {
   dfree(d_perm0_);
   for (auto& t : d_tasks_) dfree(t);
   dfree(d_info_);
+  dfree(d_df_fwd_);
+  dfree(d_df_bwd_);
+  dfree(d_df_fend_);
+  dfree(d_df_bend_);
+  dfree(d_df_tgt_);
+  dfree(d_df_dcnt0_);
+  dfree(d_df_rcnt0_);
+  dfree(d_invb_);
+  dfree(d_df_state_);
   dfree(d_acolptr_);
   dfree(d_arowind_);
   dfree(d_acolidx_);
@@ -170,6 +179,20 @@ void Engine::ensure_device() {
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
   d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
   d_info_ = dalloc<int>(1);
+  if (plan_.df_ok) {
+    d_df_fwd_ = dupload(plan_.df_fwd, s);
+    d_df_bwd_ = dupload(plan_.df_bwd, s);
+    d_df_fend_ = dupload(plan_.df_fwd_end, s);
+    d_df_bend_ = dupload(plan_.df_bwd_end, s);
+    d_df_tgt_ = dupload(plan_.df_tgt, s);
+    d_df_dcnt0_ = dupload(plan_.df_dcnt0, s);
+    d_df_rcnt0_ = dupload(plan_.df_rcnt0, s);
+    d_invb_ = dupload(plan_.inv_base, s);
+    d_df_state_ = dalloc<int>(3 + 4 * (size_t)plan_.nsuper);
+    int cus = 0;
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, opt_.device));
+    df_grid_ = std::max(1, 2 * cus);
+  }
   d_acolptr_ = dupload(h_colptr_, s);
   d_arowind_ = dupload(h_rowind_, s);
   d_acolidx_ = dupload(colidx, s);
@@ -587,8 +610,45 @@ void Engine::launch_sdiag(int bucket, unsigned g, const Task4* t, const T* arena
   }
 }
 
+// Dependency-driven sweeps: reset the counters, one persistent forward
+// launch, (LDL) the D^{-1} scaling, one persistent backward launch.
+template <typename T>
+void Engine::solve_sequence_df(const T* arena, T* Y, T* X, int nrhs) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const int ns = plan_.nsuper;
+  dev::DfState st;
+  st.head = d_df_state_;
+  st.dcnt = d_df_state_ + 2;
+  st.dflag = st.dcnt + ns;
+  st.rcnt = st.dflag + ns;
+  st.bflag = st.rcnt + ns;
+  st.abort_ = st.bflag + ns;
+  dev::k_df_reset<<<grid_for(ns), 256, 0, stream_>>>(st, d_df_dcnt0_, d_df_rcnt0_, ns);
+  const int nf = (int)plan_.df_fwd.size(), nb = (int)plan_.df_bwd.size();
+  launch_begin(3, nf);
+  dev::k_fsolve_df<T><<<(unsigned)std::min(df_grid_, nf), 256, 0, stream_>>>(
+      sn, d_df_fwd_, d_df_fend_, nf, d_df_tgt_, arena, (const T*)d_dinv_, d_lindx0_, d_invb_, Y, X,
+      nrhs, st);
+  launch_end(3);
+  if (opt_.ldl) {
+    launch_begin(3);
+    dev::k_scale_d<T><<<(unsigned)ns, 256, 0, stream_>>>(sn, (const Task4*)d_tasks_[TA_SDIAG], arena, X, nrhs);
+    launch_end(3);
+  }
+  launch_begin(6, nb);
+  dev::k_bsolve_df<T><<<(unsigned)std::min(df_grid_, nb), 256, 0, stream_>>>(
+      sn, d_df_bwd_, d_df_bend_, nb, d_df_tgt_, arena, (const T*)d_dinv_, d_lindx0_, d_invb_, X, nrhs,
+      st);
+  launch_end(6);
+  HIP_OK(hipGetLastError());
+}
+
 template <typename T>
 void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
+  if (use_df()) {
+    solve_sequence_df<T>(arena, Y, X, nrhs);
+    return;
+  }
   const SnInfo* sn = (const SnInfo*)d_sn_;
   const bool ldl = opt_.ldl != 0;
   const Task4* sd = (const Task4*)d_tasks_[TA_SDIAG];
@@ -795,10 +855,14 @@ void Engine::pcg_restart(int nrhs) {
 
 SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   if (!factored_) throw std::logic_error("Solve called without a successful Factorize");
+  if (opt_.solve_dataflow)
+    throw std::logic_error("the dependency-driven solve (solve_dataflow) is disabled in this build");
   if (nrhs <= 0) throw std::invalid_argument("nrhs must be positive");
   HIP_OK(hipSetDevice(opt_.device));
   auto t0 = std::chrono::steady_clock::now();
   ensure_solve_capacity(nrhs);
+  int* df_abort = use_df() ? d_df_state_ + 2 + 4 * (size_t)plan_.nsuper : nullptr;
+  if (df_abort) dev::k_fill_zero<int><<<1, 64, 0, stream_>>>(df_abort, 1);
   const size_t cnt = (size_t)n_ * nrhs;
   HIP_OK(hipMemcpyAsync(d_b_, rhs, cnt * sizeof(double),
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
@@ -874,8 +938,12 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
       }
     }
   }
+  int aborted = 0;
+  if (df_abort) HIP_OK(hipMemcpyAsync(&aborted, df_abort, sizeof(int), hipMemcpyDeviceToHost, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
   collect_events();
+  if (aborted)
+    throw std::runtime_error("dependency-driven solve: a wait exceeded its budget (plan inconsistent)");
   last_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   last_report = rep;
   return rep;

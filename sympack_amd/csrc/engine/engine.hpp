@@ -24,6 +24,7 @@ struct EngineOptions {
   int profile = 0;        // record HIP events around every launch
   int verbose = 0;
   int merge_updates = 1;  // 0 off, 1 on levels that fill the GPU, 2 every qualifying chain child
+  int solve_dataflow = 0; // dependency-driven solve sweeps: disabled, requesting it is an error
   std::vector<Int> user_perm;
 };
 
@@ -101,6 +102,11 @@ class Engine {
   template <typename T>
   void solve_sequence(const T* arena, T* Y, T* X, int nrhs);
   template <typename T>
+  void solve_sequence_df(const T* arena, T* Y, T* X, int nrhs);
+  // The dependency-driven sweeps did not finish on MI355X (waits on counters
+  // updated from other XCDs), so they are switched off: see solve().
+  bool use_df() const { return false; }
+  template <typename T>
   void assemble(T* arena);
   void residual(const double* b, const double* x, double* r, int nrhs);
   void precond(const double* r, double* z, int nrhs);
@@ -159,6 +165,17 @@ class Engine {
   int32_t* d_perm0_ = nullptr;
   void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm
   int* d_info_ = nullptr;
+  // dependency-driven solve
+  Task4* d_df_fwd_ = nullptr;
+  Task4* d_df_bwd_ = nullptr;
+  int32_t* d_df_fend_ = nullptr;
+  int32_t* d_df_bend_ = nullptr;
+  int32_t* d_df_tgt_ = nullptr;
+  int32_t* d_df_dcnt0_ = nullptr;
+  int32_t* d_df_rcnt0_ = nullptr;
+  int64_t* d_invb_ = nullptr;
+  int* d_df_state_ = nullptr;  // head[2], dcnt, dflag, rcnt, bflag, abort
+  int df_grid_ = 0;
   int64_t* d_acolptr_ = nullptr;
   int32_t* d_arowind_ = nullptr;
   int32_t* d_acolidx_ = nullptr;

@@ -1002,27 +1002,23 @@ __global__ __launch_bounds__(NT) void k_sdiag3(const SnInfo* __restrict__ sn,
 // recurrence as k_sdiag3, which handles each right-hand-side group with one
 // wave.  Task {sn, inverse-tile base, -, -}; Y may alias X.
 template <typename T, bool BWD, int NBM>
-__global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
-                                               const Task4* __restrict__ tasks,
-                                               const T* __restrict__ arena, const T* __restrict__ dinv,
-                                               const T* Y, T* X, int nrhs) {
+__device__ __forceinline__ void sdiag4_body(const SnInfo& s, int64_t invb,
+                                            const T* __restrict__ arena, const T* __restrict__ dinv,
+                                            const T* Y, T* X, int nrhs, int j0, T (*xs)[4][64]) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NO = (NBM + 3) / 4;  // blocks per wave
-  __shared__ __attribute__((aligned(16))) T xs[NBM][4][64];
-  const Task4 t = tasks[blockIdx.x];
-  const SnInfo s = sn[t.a];
   const int w = s.w;
   const int nbt = (w + SUB - 1) / SUB;
   const T* __restrict__ P = arena + s.off;
   const int64_t ld = s.r;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const T* __restrict__ Dv = dinv + (int64_t)t.b * (SUB * SUB);
+  const T* __restrict__ Dv = dinv + invb * (SUB * SUB);
   const int m16 = lane & 15;
   // Loads in the order the recurrence consumes them (right-hand sides,
   // inverse tiles, then the fold operands block by block), so the first
   // steps wait only for the first loads.
-  const int j = lane & 15;
+  const int j = j0 + (lane & 15);
   const int jcl = min(j, nrhs - 1);
   const T jm = T(j < nrhs);
   const int64_t base = (int64_t)s.fc * nrhs + jcl;
@@ -1091,18 +1087,30 @@ __global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
       }
     }
   }
-  if (j >= nrhs) return;
+  if (j < nrhs) {
 #pragma unroll
-  for (int i = 0; i < NO; ++i) {
-    const int K = wave + 4 * i;
-    if (K < nbt) {
+    for (int i = 0; i < NO; ++i) {
+      const int K = wave + 4 * i;
+      if (K < nbt) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = SUB * K + Mfma<T>::row(lane, v);
-        if (row < w) X[base + (int64_t)row * nrhs] = xo[i][v];
+        for (int v = 0; v < 4; ++v) {
+          const int row = SUB * K + Mfma<T>::row(lane, v);
+          if (row < w) X[base + (int64_t)row * nrhs] = xo[i][v];
+        }
       }
     }
   }
+  __syncthreads();  // xs is reused by the next call
+}
+
+template <typename T, bool BWD, int NBM>
+__global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
+                                               const Task4* __restrict__ tasks,
+                                               const T* __restrict__ arena, const T* __restrict__ dinv,
+                                               const T* Y, T* X, int nrhs) {
+  __shared__ __attribute__((aligned(16))) T xs[NBM][4][64];
+  const Task4 t = tasks[blockIdx.x];
+  sdiag4_body<T, BWD, NBM>(sn[t.a], t.b, arena, dinv, Y, X, nrhs, 0, xs);
 }
 
 // Multi-RHS off-diagonal solve updates on MFMA (used for nrhs >= 4), one
@@ -1111,20 +1119,15 @@ __global__ __launch_bounds__(NT) void k_sdiag4(const SnInfo* __restrict__ sn,
 // blockIdx.y = right-hand-side chunk.
 // Forward:  Y[row(a)] -= sum_k L(a,k) X_I(k)      (one atomic per output)
 template <typename T>
-__global__ __launch_bounds__(NT) void k_soffm_fwd(const SnInfo* __restrict__ sn,
-                                                   const Task4* __restrict__ tasks,
-                                                   const T* __restrict__ arena,
-                                                   const int32_t* __restrict__ lindx0,
-                                                   const T* __restrict__ X, T* __restrict__ Y,
-                                                   int nrhs) {
+__device__ __forceinline__ void soffm_fwd_body(const SnInfo& s, int rt, const T* __restrict__ arena,
+                                               const int32_t* __restrict__ lindx0,
+                                               const T* __restrict__ X, T* __restrict__ Y, int nrhs,
+                                               int j0) {
   typedef typename Mfma<T>::v4 v4;
-  const Task4 t = tasks[blockIdx.x];
-  const SnInfo s = sn[t.a];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = s.r - s.w;
-  const int a0 = t.b * TILE + 16 * wave;
+  const int a0 = rt * TILE + 16 * wave;
   if (a0 >= m) return;
-  const int j0 = blockIdx.y * SUB;
   const int mi = lane & 15, g = lane >> 4;
   const int ac = min(a0 + mi, m - 1);
   const T amask = T(a0 + mi < m);
@@ -1157,22 +1160,28 @@ __global__ __launch_bounds__(NT) void k_soffm_fwd(const SnInfo* __restrict__ sn,
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(NT) void k_soffm_fwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   const int32_t* __restrict__ lindx0,
+                                                   const T* __restrict__ X, T* __restrict__ Y,
+                                                   int nrhs) {
+  const Task4 t = tasks[blockIdx.x];
+  soffm_fwd_body<T>(sn[t.a], t.b, arena, lindx0, X, Y, nrhs, blockIdx.y * SUB);
+}
+
 // Backward:  X_I(k) -= sum_{a in tile} L(a,k) Y[row(a)]   (partial sums over
 // 64 rows, atomics into X_I).  The reduction index is permuted so that lane
 // (m, g) reads rows 16g..16g+15 of one column: one cache line per lane.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
-                                                   const Task4* __restrict__ tasks,
-                                                   const T* __restrict__ arena,
-                                                   const int32_t* __restrict__ lindx0,
-                                                   T* __restrict__ X, int nrhs) {
+__device__ __forceinline__ void soffm_bwd_body(const SnInfo& s, int rt, const T* __restrict__ arena,
+                                               const int32_t* __restrict__ lindx0, T* __restrict__ X,
+                                               int nrhs, int j0) {
   typedef typename Mfma<T>::v4 v4;
-  const Task4 t = tasks[blockIdx.x];
-  const SnInfo s = sn[t.a];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = s.r - s.w;
-  const int a0 = t.b * TILE;
-  const int j0 = blockIdx.y * SUB;
+  const int a0 = rt * TILE;
   const int mi = lane & 15, g = lane >> 4;
   const int jn = j0 + mi;
   const int jc = min(jn, nrhs - 1);
@@ -1220,6 +1229,170 @@ __global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
           }
         }
       }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
+                                                   const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ arena,
+                                                   const int32_t* __restrict__ lindx0,
+                                                   T* __restrict__ X, int nrhs) {
+  const Task4 t = tasks[blockIdx.x];
+  soffm_bwd_body<T>(sn[t.a], t.b, arena, lindx0, X, nrhs, blockIdx.y * SUB);
+}
+
+// ---------------------------------------------------------------------------
+// Dependency-driven solve (one persistent launch per sweep, no level
+// barriers).  Workgroups take tasks in a topological order from a global
+// ticket counter; a task waits only for the tasks it depends on, which hold
+// smaller tickets and so were taken by workgroups already running: every
+// wait ends.  Waits are bounded all the same: a wait that exceeds its
+// budget raises the shared abort flag, after which every wait returns at
+// once and the host reports the error (the result is then not used).
+//
+// Forward, supernodes in postorder: D(s) = diagonal solve of s, waiting for
+// dcnt[s] = 0 (the off-diagonal tasks that update rows of s); then O(s, rt)
+// for each 64-row tile of s, waiting for D(s), which subtract L_rt X_s from
+// Y and count down dcnt of every supernode owning rows of the tile.
+// Backward, supernodes in reverse postorder: R(s, rt) waits until every
+// supernode owning rows of the tile is solved (bflag), subtracts
+// L_rt^T X[rows] from X_s and counts down rcnt[s]; then B(s) waits for
+// rcnt[s] = 0 and solves the diagonal block.
+// The L2s of the XCDs are not coherent with each other: data written before
+// a signal is released at agent scope (L2 write-back) and acquired by the
+// waiter (L2 invalidate); the counters, flags and tickets themselves are
+// read and updated with system-scope atomics, which bypass the XCD caches (a
+// device-scope load may be served by the waiter's own L2 and never see
+// another XCD's update).  Each wait is bounded in time (1 s of the 100 MHz
+// real-time counter).
+// Task {kind (0 D / 1 O / 2 R / 3 B), sn, row tile, first target}, targets
+// tgt[first .. tend[task]).
+// ---------------------------------------------------------------------------
+struct DfState {
+  int* head;     // [2] ticket counters (forward, backward)
+  int* dcnt;     // per supernode: forward updates still to come
+  int* dflag;    // per supernode: forward diagonal solve done
+  int* rcnt;     // per supernode: backward row tiles still to come
+  int* bflag;    // per supernode: backward diagonal solve done
+  int* abort_;   // set when a wait exceeded its budget
+};
+
+__device__ __forceinline__ int ld_sys(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int add_sys(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// thread 0 waits until *p == want (or abort); every thread then acquires
+__device__ __forceinline__ void df_wait(const int* p, int want, int* abort_) {
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    while (ld_sys(p) != want) {
+      if (ld_sys(abort_)) break;
+      __builtin_amdgcn_s_sleep(8);
+      if (__builtin_amdgcn_s_memrealtime() - t0 > 100000000ull) {
+        st_sys(abort_, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// every thread releases its writes, then thread 0 signals
+__device__ __forceinline__ void df_release() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+}
+
+__device__ __forceinline__ int df_ticket(int* head, int* slot) {
+  if (threadIdx.x == 0) *slot = add_sys(head, 1);
+  __syncthreads();
+  const int t = *slot;
+  __syncthreads();
+  return t;
+}
+
+// counters for one solve (the abort flag is cleared by the host per Solve)
+__global__ void k_df_reset(DfState st, const int* __restrict__ dcnt0, const int* __restrict__ rcnt0,
+                           int ns) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+    st.dcnt[i] = dcnt0[i];
+    st.rcnt[i] = rcnt0[i];
+    st.dflag[i] = 0;
+    st.bflag[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) st.head[threadIdx.x] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fsolve_df(const SnInfo* __restrict__ sn,
+                                                  const Task4* __restrict__ tasks,
+                                                  const int32_t* __restrict__ tend, int ntasks,
+                                                  const int32_t* __restrict__ tgt,
+                                                  const T* __restrict__ arena, const T* __restrict__ dinv,
+                                                  const int32_t* __restrict__ lindx0,
+                                                  const int64_t* __restrict__ invb, T* Y, T* X,
+                                                  int nrhs, DfState st) {
+  constexpr int NBM = 10;
+  __shared__ __attribute__((aligned(16))) T xs[NBM][4][64];
+  __shared__ int slot;
+  for (;;) {
+    const int ti = df_ticket(st.head, &slot);
+    if (ti >= ntasks) break;
+    const Task4 t = tasks[ti];
+    const SnInfo s = sn[t.b];
+    if (t.a == 0) {
+      df_wait(st.dcnt + t.b, 0, st.abort_);
+      for (int j0 = 0; j0 < nrhs; j0 += SUB)
+        sdiag4_body<T, false, NBM>(s, invb[t.b], arena, dinv, Y, X, nrhs, j0, xs);
+      df_release();
+      if (threadIdx.x == 0) st_sys(st.dflag + t.b, 1);
+    } else {
+      df_wait(st.dflag + t.b, 1, st.abort_);
+      for (int j0 = 0; j0 < nrhs; j0 += SUB) soffm_fwd_body<T>(s, t.c, arena, lindx0, X, Y, nrhs, j0);
+      df_release();
+      if (threadIdx.x == 0)
+        for (int q = t.d; q < tend[ti]; ++q) add_sys(st.dcnt + tgt[q], -1);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_bsolve_df(const SnInfo* __restrict__ sn,
+                                                  const Task4* __restrict__ tasks,
+                                                  const int32_t* __restrict__ tend, int ntasks,
+                                                  const int32_t* __restrict__ tgt,
+                                                  const T* __restrict__ arena, const T* __restrict__ dinv,
+                                                  const int32_t* __restrict__ lindx0,
+                                                  const int64_t* __restrict__ invb, T* X, int nrhs,
+                                                  DfState st) {
+  constexpr int NBM = 10;
+  __shared__ __attribute__((aligned(16))) T xs[NBM][4][64];
+  __shared__ int slot;
+  for (;;) {
+    const int ti = df_ticket(st.head + 1, &slot);
+    if (ti >= ntasks) break;
+    const Task4 t = tasks[ti];
+    const SnInfo s = sn[t.b];
+    if (t.a == 2) {
+      for (int q = t.d; q < tend[ti]; ++q) df_wait(st.bflag + tgt[q], 1, st.abort_);
+      for (int j0 = 0; j0 < nrhs; j0 += SUB) soffm_bwd_body<T>(s, t.c, arena, lindx0, X, nrhs, j0);
+      df_release();
+      if (threadIdx.x == 0) add_sys(st.rcnt + t.b, -1);
+    } else {
+      df_wait(st.rcnt + t.b, 0, st.abort_);
+      for (int j0 = 0; j0 < nrhs; j0 += SUB)
+        sdiag4_body<T, true, NBM>(s, invb[t.b], arena, dinv, X, X, nrhs, j0, xs);
+      df_release();
+      if (threadIdx.x == 0) st_sys(st.bflag + t.b, 1);
     }
   }
 }

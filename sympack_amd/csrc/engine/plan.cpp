@@ -305,6 +305,50 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
     mcount[L] = (int64_t)p.soffm_tasks.size() - mfirst[L];
   }
+  // dependency-driven solve task lists
+  p.df_ok = p.max_w <= kStep;
+  if (p.df_ok) {
+    p.df_dcnt0.assign(ns, 0);
+    p.df_rcnt0.assign(ns, 0);
+    std::vector<int32_t> last_seen(ns, -1);
+    std::vector<int64_t> dtask(ns, 0);  // forward D task of every supernode
+    int32_t stamp = 0;
+    for (Int I = 0; I < ns; ++I) {
+      const SnInfo& si = p.sn[I];
+      const int32_t m = si.r - si.w;
+      const int32_t nt = (int32_t)ceil_div(m, kTile);
+      p.df_rcnt0[I] = nt;
+      dtask[I] = (int64_t)p.df_fwd.size();
+      p.df_fwd.push_back({0, I, 0, 0});
+      p.df_fwd_end.push_back(0);
+      for (int32_t rt = 0; rt < nt; ++rt) {
+        const int32_t first = (int32_t)p.df_tgt.size();
+        ++stamp;
+        for (int32_t a = rt * kTile; a < std::min<int32_t>(m, (rt + 1) * kTile); ++a) {
+          const int32_t T = row_sup[p.lindx0[si.lx + si.w + a]];
+          if (last_seen[T] != stamp) {
+            last_seen[T] = stamp;
+            p.df_tgt.push_back(T);
+            ++p.df_dcnt0[T];
+          }
+        }
+        p.df_fwd.push_back({1, I, rt, first});
+        p.df_fwd_end.push_back((int32_t)p.df_tgt.size());
+      }
+    }
+    // backward: the same tiles in reverse postorder, waiting for the owners
+    // of their rows
+    for (Int I = ns - 1; I >= 0; --I) {
+      for (int32_t rt = 0; rt < p.df_rcnt0[I]; ++rt) {
+        const int64_t q = dtask[I] + 1 + rt;
+        p.df_bwd.push_back({2, I, rt, p.df_fwd[q].d});
+        p.df_bwd_end.push_back(p.df_fwd_end[q]);
+      }
+      p.df_bwd.push_back({3, I, 0, 0});
+      p.df_bwd_end.push_back(0);
+    }
+  }
+
   // forward: levels bottom-up, diagonal solves then off-diagonal updates.
   std::vector<int> dbucket(p.nlevels, 1);
   for (int L = 0; L < p.nlevels; ++L) {
@@ -325,6 +369,61 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     d.aux = dbucket[L];
     p.bwd_seq.push_back(d);
   }
+}
+
+std::string validate_solve_plan(const Plan& p) {
+  if (!p.df_ok) return std::string();
+  const Int ns = p.nsuper;
+  std::vector<int32_t> dcnt = p.df_dcnt0, rcnt = p.df_rcnt0;
+  std::vector<char> dflag(ns, 0), bflag(ns, 0);
+  std::vector<int32_t> row_sup(p.n, -1);
+  for (Int I = 0; I < ns; ++I)
+    for (int32_t c = 0; c < p.sn[I].w; ++c) row_sup[p.sn[I].fc + c] = I;
+  auto owners_ok = [&](const Task4& t, int32_t end) -> bool {
+    const SnInfo& si = p.sn[t.b];
+    const int32_t m = si.r - si.w;
+    std::vector<int32_t> want;
+    for (int32_t a = t.c * kTile; a < std::min<int32_t>(m, (t.c + 1) * kTile); ++a) {
+      const int32_t T = row_sup[p.lindx0[si.lx + si.w + a]];
+      if (std::find(want.begin(), want.end(), T) == want.end()) want.push_back(T);
+    }
+    std::vector<int32_t> got(p.df_tgt.begin() + t.d, p.df_tgt.begin() + end);
+    std::sort(want.begin(), want.end());
+    std::sort(got.begin(), got.end());
+    return want == got;
+  };
+  for (size_t q = 0; q < p.df_fwd.size(); ++q) {
+    const Task4& t = p.df_fwd[q];
+    if (t.a == 0) {
+      if (dcnt[t.b] != 0) return "forward D(" + std::to_string(t.b) + ") reached with pending updates";
+      dflag[t.b] = 1;
+    } else if (t.a == 1) {
+      if (!dflag[t.b]) return "forward O(" + std::to_string(t.b) + ") before its diagonal solve";
+      if (!owners_ok(t, p.df_fwd_end[q])) return "forward O(" + std::to_string(t.b) + ") targets";
+      for (int32_t k = t.d; k < p.df_fwd_end[q]; ++k) --dcnt[p.df_tgt[k]];
+    } else {
+      return "bad forward task kind";
+    }
+  }
+  for (Int I = 0; I < ns; ++I)
+    if (dcnt[I] != 0 || !dflag[I]) return "forward counters not drained";
+  for (size_t q = 0; q < p.df_bwd.size(); ++q) {
+    const Task4& t = p.df_bwd[q];
+    if (t.a == 2) {
+      for (int32_t k = t.d; k < p.df_bwd_end[q]; ++k)
+        if (!bflag[p.df_tgt[k]]) return "backward R(" + std::to_string(t.b) + ") before an owner is solved";
+      if (!owners_ok(t, p.df_bwd_end[q])) return "backward R(" + std::to_string(t.b) + ") targets";
+      --rcnt[t.b];
+    } else if (t.a == 3) {
+      if (rcnt[t.b] != 0) return "backward B(" + std::to_string(t.b) + ") reached with pending tiles";
+      bflag[t.b] = 1;
+    } else {
+      return "bad backward task kind";
+    }
+  }
+  for (Int I = 0; I < ns; ++I)
+    if (rcnt[I] != 0 || !bflag[I]) return "backward counters not drained";
+  return std::string();
 }
 
 }  // namespace spk

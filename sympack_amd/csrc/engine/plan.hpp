@@ -13,6 +13,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "../analysis/analysis.hpp"
@@ -99,6 +100,18 @@ struct Plan {
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
   std::vector<Task4> soffm_tasks;  // {sn, row_tile, 0, 0}: 64-row tiles of the off-diagonal panel
   std::vector<LaunchRange> fwd_seq, bwd_seq;  // kind 3/6 aux: diag_bucket of the level, 0 = w > kStep
+  // dependency-driven solve (one persistent launch per sweep), available when
+  // every supernode fits one diagonal block (w <= kStep).  Tasks
+  // {kind, sn, row tile, first target}: forward D(s) = 0 / O(s,rt) = 1 in
+  // postorder, backward R(s,rt) = 2 / B(s) = 3 in reverse postorder; the
+  // targets of O(s,rt) / R(s,rt), df_tgt[d .. end), are the supernodes
+  // owning the tile's rows.
+  bool df_ok = false;
+  std::vector<Task4> df_fwd, df_bwd;
+  std::vector<int32_t> df_fwd_end, df_bwd_end;
+  std::vector<int32_t> df_tgt;          // shared by O and R tasks (same tiles)
+  std::vector<int32_t> df_dcnt0;        // per supernode: O tasks that update its rows
+  std::vector<int32_t> df_rcnt0;        // per supernode: its row tiles
   // algorithmic counts
   double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
   int64_t schur_entries = 0;       // lower-triangle entries scattered
@@ -111,5 +124,13 @@ struct Plan {
 // 2 merge every qualifying chain child.
 void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
                 Plan& p, int merge_updates = 1);
+
+// Checks the dependency-driven solve lists by running them in ticket order
+// on one worker: every wait must already be satisfied when its task is
+// reached (so with any number of workers taking tickets in order, the
+// oldest unfinished task can always proceed), every counter must reach zero
+// and every task's targets must be the owners of its rows.  Returns an
+// empty string or the first violation.
+std::string validate_solve_plan(const Plan& p);
 
 }  // namespace spk

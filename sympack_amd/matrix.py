@@ -50,7 +50,7 @@ class symPACKOptions:
 
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
-                 profile=False, device=0, merge_updates=1):
+                 profile=False, device=0, merge_updates=1, solve_dataflow=False):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -62,6 +62,7 @@ class symPACKOptions:
         self.profile = profile
         self.device = device
         self.merge_updates = merge_updates
+        self.solve_dataflow = solve_dataflow
 
     @classmethod
     def from_args(cls, args):
@@ -77,7 +78,7 @@ class symPACKOptions:
         return cls(ordering=inv[o.ordering], decomposition="LDL" if o.decomposition else "LL",
                    relax=r, precision="FP32_IR" if o.precision else "FP64",
                    ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
-                   merge_updates=int(o.merge_updates))
+                   merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow))
 
     def to_c(self):
         lib = _lib.load()
@@ -99,6 +100,7 @@ class symPACKOptions:
         o.profile = 1 if self.profile else 0
         o.device = self.device
         o.merge_updates = int(self.merge_updates)
+        o.solve_dataflow = 1 if self.solve_dataflow else 0
         return o
 
 
@@ -331,6 +333,10 @@ class symPACKMatrix2D:
         d = np.zeros(self.TotalSupernodeCnt(), np.int32)
         _lib.check(self._lib.spk_get_update_merges(self._h, _ptr(d, ctypes.c_int32)), "merges")
         return d
+
+    def ValidateSolvePlan(self):
+        """Raise if the dependency-driven solve's task lists are inconsistent."""
+        _lib.check(self._lib.spk_validate_solve_plan(self._h), "solve plan")
 
     def GetCells(self):
         nc = ctypes.c_int64()

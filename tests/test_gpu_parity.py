@@ -290,3 +290,17 @@ def test_ldl_48cube_refactorizations_match_oracle():
         be = onum.backward_error(A, Mx.GetSolution(2), B)
         print("refactor", rep, "berr %.2e oracle %.2e" % (be, be_o), flush=True)
         assert be <= 10 * max(be_o, 1e-16)
+
+
+def test_disabled_dependency_driven_solve_is_rejected():
+    """solve_dataflow=1 names the dependency-driven sweeps, which are switched
+    off in this build: Solve raises before launching anything."""
+    n, cp, ri, v = M.laplacian_2d_5pt(16)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="MMD", solve_dataflow=True))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    Mx.Factorize()
+    with pytest.raises(ValueError):
+        Mx.Solve(np.ones(n), 1)

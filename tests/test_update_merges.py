@@ -91,3 +91,11 @@ def test_auto_merging_only_on_wide_levels():
     # the top of the tree (levels with a single supernode) merges nothing
     top = lev.max()
     assert np.all(d[lev >= top - 5] == -1)
+
+
+@pytest.mark.parametrize("name,gen,maxsize", CASES + [("diff3d_48_w150", lambda: M.diffusion_3d_7pt(48, seed=0), 150)],
+                         ids=[c[0] for c in CASES] + ["diff3d_48_w150"])
+def test_dependency_driven_solve_plan(name, gen, maxsize):
+    """The solve's task lists: in ticket order no wait blocks, every counter
+    drains and every tile's targets are the owners of its rows."""
+    _plan(gen, maxsize).ValidateSolvePlan()
