@@ -301,7 +301,7 @@ def main():
         M.Factorize()
         prof_ms.append((time.perf_counter() - a) * 1e3)
     M.set_profile(False)
-    kinds = {k: M.kernel_stats(k) for k in range(8)}
+    kinds = {k: M.kernel_stats(k) for k in range(10)}
 
     # solve latency distribution (factor resident, repeated solves)
     lat = []

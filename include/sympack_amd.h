@@ -88,8 +88,9 @@ typedef struct spk_stats {
 } spk_stats;
 
 /* Per-kernel-kind event timings (profile mode).  kind: 0 diagonal factor,
- * 7 panel TRSM, 1 in-panel update, 2 Schur update, 3 forward diagonal,
- * 4 forward off-diagonal, 5 backward off-diagonal, 6 backward diagonal. */
+ * 7 panel TRSM, 9 fused diagonal factor + panel TRSM, 1 in-panel update,
+ * 2 Schur update, 3 forward diagonal, 4 forward off-diagonal, 5 backward
+ * off-diagonal, 6 backward diagonal. */
 int spk_get_kernel_stats(spk_handle h, int kind, int64_t* launches, double* ms);
 int spk_reset_kernel_stats(spk_handle h);
 /* Switch per-launch event timing on/off after spk_init (no reference

@@ -314,7 +314,7 @@ int spk_get_stats(spk_handle h, spk_stats* st) {
     st->diag_flops = p.diag_flops;
     st->schur_entries = p.schur_entries;
     const auto& ks = h->engine.kind_stats;
-    st->launches_factor = ks[0].launches + ks[1].launches + ks[2].launches + ks[7].launches;
+    st->launches_factor = ks[0].launches + ks[1].launches + ks[2].launches + ks[7].launches + ks[9].launches;
     st->launches_solve = ks[3].launches + ks[4].launches + ks[5].launches + ks[6].launches;
     st->factor_ms = h->engine.last_factor_ms;
     st->solve_ms = h->engine.last_solve_ms;

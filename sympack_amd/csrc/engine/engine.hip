@@ -47,10 +47,10 @@ inline unsigned grid_for(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
 }
-enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM };
+enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED };
 }  // namespace
 
-Engine::Engine() { kind_stats.resize(8); }
+Engine::Engine() { kind_stats.resize(10); }
 
 Engine::~Engine() {
   try {
@@ -76,6 +76,7 @@ void Engine::free_device() {
   dfree(d_perm0_);
   for (auto& t : d_tasks_) dfree(t);
   dfree(d_info_);
+  dfree(d_fcnt_);
   dfree(d_df_fwd_);
   dfree(d_df_bwd_);
   dfree(d_df_fend_);
@@ -115,7 +116,7 @@ int64_t Engine::device_bytes(int nrhs) const {
   auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
   const Plan& p = plan_;
   int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
-              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) +
+              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) + vb(p.fused_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
@@ -176,9 +177,14 @@ void Engine::ensure_device() {
   d_tasks_[TA_TRSM] = dupload(plan_.trsm_tasks, s);
   d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
   d_tasks_[TA_SCHUR] = dupload(plan_.schur_tasks, s);
+  d_tasks_[TA_FUSED] = dupload(plan_.fused_tasks, s);
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
   d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
   d_info_ = dalloc<int>(1);
+  if (!plan_.fused_tasks.empty()) {  // load counters of the fused block factor + TRSM
+    d_fcnt_ = dalloc<int>((size_t)plan_.nsuper);
+    HIP_OK(hipMemset(d_fcnt_, 0, sizeof(int) * (size_t)plan_.nsuper));
+  }
   if (plan_.df_ok) {
     d_df_fwd_ = dupload(plan_.df_fwd, s);
     d_df_bwd_ = dupload(plan_.df_bwd, s);
@@ -466,6 +472,35 @@ void Engine::launch_ptrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* a
 #undef SPK_PTRSM
 }
 
+template <typename T, bool LDL, int NBM>
+static void diagtrsm_launch(unsigned g, hipStream_t st, const SnInfo* sn, const Task4* t, T* arena,
+                            T* dinv, int* info, int* cnt) {
+  dev::k_diagtrsm<T, LDL, NBM><<<g, 256, 0, st>>>(sn, t, arena, dinv, info, cnt);
+}
+
+template <typename T>
+void Engine::launch_diagtrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  T* dinv = (T*)d_dinv_;
+#define SPK_FUSED(NBM)                                                                   \
+  case NBM:                                                                              \
+    if (ldl)                                                                             \
+      diagtrsm_launch<T, true, NBM>(g, stream_, sn, t, arena, dinv, d_info_, d_fcnt_);  \
+    else                                                                                 \
+      diagtrsm_launch<T, false, NBM>(g, stream_, sn, t, arena, dinv, d_info_, d_fcnt_); \
+    break;
+  switch (bucket) {
+    SPK_FUSED(1)
+    SPK_FUSED(2)
+    SPK_FUSED(4)
+    SPK_FUSED(6)
+    SPK_FUSED(10)
+    default:
+      throw std::logic_error("bad diagonal-block bucket");
+  }
+#undef SPK_FUSED
+}
+
 template <typename T>
 void Engine::factor_sequence(T* arena) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
@@ -483,6 +518,11 @@ void Engine::factor_sequence(T* arena) {
       case 7: {
         const Task4* t = (const Task4*)d_tasks_[TA_TRSM] + r.first;
         launch_ptrsm<T>(r.aux, ldl, g, t, arena);
+        break;
+      }
+      case 9: {
+        const Task4* t = (const Task4*)d_tasks_[TA_FUSED] + r.first;
+        launch_diagtrsm<T>(r.aux, ldl, g, t, arena);
         break;
       }
       case 1: {

@@ -48,7 +48,7 @@ class Engine {
 
   void init(const EngineOptions& o);
   void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
-  void set_profile(bool on, unsigned mask = 0xffu) {
+  void set_profile(bool on, unsigned mask = 0x3ffu) {
     if ((on ? 1 : 0) != opt_.profile || mask != profile_mask_) destroy_graphs();
     opt_.profile = on ? 1 : 0;
     profile_mask_ = mask;
@@ -99,6 +99,8 @@ class Engine {
   void launch_sdiag(int bucket, unsigned g, const Task4* t, const T* arena, const T* Y, T* X, int nrhs);
   template <typename T>
   void launch_ptrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
+  template <typename T>
+  void launch_diagtrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
   template <typename T>
   void solve_sequence(const T* arena, T* Y, T* X, int nrhs);
   template <typename T>
@@ -163,8 +165,9 @@ class Engine {
   int32_t* d_mmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
   int32_t* d_perm0_ = nullptr;
-  void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm
+  void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused
   int* d_info_ = nullptr;
+  int* d_fcnt_ = nullptr;  // per supernode: loads of the fused block factor + TRSM
   // dependency-driven solve
   Task4* d_df_fwd_ = nullptr;
   Task4* d_df_bwd_ = nullptr;
@@ -206,7 +209,7 @@ class Engine {
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
   std::vector<int64_t> ev_grid_;
-  unsigned profile_mask_ = 0xffu;  // kinds whose launches get events
+  unsigned profile_mask_ = 0x3ffu;  // kinds whose launches get events
   size_t ev_next_ = 0;
   void collect_events();
 };

@@ -207,19 +207,32 @@ __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 +
 // {sn, step, inverse-tile offset, -}; info receives the smallest failing
 // pivot column (1-based, permuted order).
 // ---------------------------------------------------------------------------
-template <typename T, bool LDL, int NBM>
-__global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
-                                              const Task4* __restrict__ tasks,
-                                              T* __restrict__ arena, T* __restrict__ dinv,
-                                              int* __restrict__ info) {
+// LDS of the block factor (and of the fused block factor + TRSM)
+template <typename T, int NBM>
+struct DiagLds {
+  T tl[NBM * (NBM + 1) / 2 * TSZ];
+  T dsh[SUB];
+  T ysh[NBM * SUB];  // LL: 1/L_jj of every column; LDL (fused): 1/D_j
+  T colb[NT / 64][SUB];
+  int store;  // fused: this workgroup writes the factored block back
+};
+
+// FUSED: the nt workgroups of a supernode (one per TRSM row tile) each
+// factor the block; the inverse tiles then replace the factored diagonal
+// tiles in LDS (and 1/D of every column is kept for LDL).  The factored block
+// and the inverse tiles are written to HBM by the workgroup that is the last
+// of the nt to finish loading the block (*cnt counts the loads, and is reset
+// by that workgroup): the others have read the unfactored block by then.
+template <typename T, bool LDL, int NBM, bool FUSED>
+__device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __restrict__ arena,
+                                          T* __restrict__ dinv, int* __restrict__ info,
+                                          DiagLds<T, NBM>& L, int* __restrict__ cnt, int nt) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NTL = NBM * (NBM + 1) / 2;
-  __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
-  __shared__ T dsh[SUB];
-  __shared__ T ysh[NBM * SUB];  // LL: 1/L_jj of every column
-  __shared__ __attribute__((aligned(16))) T colb[NT / 64][SUB];
-  const Task4 t = tasks[blockIdx.x];
-  const SnInfo s = sn[t.a];
+  T* __restrict__ tl = L.tl;
+  T* __restrict__ dsh = L.dsh;
+  T* __restrict__ ysh = L.ysh;
+  T (*colb)[SUB] = L.colb;
   const int c0 = t.b * kStep;
   const int bs = min(kStep, s.w - c0);
   const int nb = (bs + SUB - 1) / SUB;
@@ -251,6 +264,12 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
       }
   }
   __syncthreads();
+  if (FUSED && tid == 0) {
+    // every load of this workgroup has returned (its values are in LDS)
+    const int prior = atomicAdd(cnt, 1);
+    L.store = (prior == nt - 1);
+    if (prior == nt - 1) atomicExch(cnt, 0);
+  }
 
   // Trailing tiles live in MFMA accumulators for the whole factorization:
   // tile t of the k-descending order belongs to wave t % 4, slot t / 4, and
@@ -395,27 +414,28 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
   // ---- inverses of the factored 16x16 diagonal tiles for the panel TRSM,
   // all at once: 16 lanes per tile, lane (c) solves L z = e_c (LDL: unit
   // diagonal) by forward substitution ----
-  {
-    const int Jt = 4 * wave + (lane >> 4);
-    const int c = lane & 15;
-    if (Jt < nb) {
-      const T* Lt = tl + tslot(Jt, Jt) * TSZ;
-      T z[SUB];
+  const bool store = FUSED ? (L.store != 0) : true;
+  const int Jt = 4 * wave + (lane >> 4);
+  const int cz = lane & 15;
+  T z[SUB];
+  if (Jt < nb) {
+    const T* Lt = tl + tslot(Jt, Jt) * TSZ;
 #pragma unroll
-      for (int i = 0; i < SUB; ++i) {
-        T acc = (c == i) ? T(1) : T(0);
+    for (int i = 0; i < SUB; ++i) {
+      T acc = (cz == i) ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc -= Lt[k * TSTR + i] * z[k];
-        z[i] = LDL ? acc : acc * ysh[SUB * Jt + i];
-      }
-      T* out = dinv + (int64_t)(t.c + Jt) * (SUB * SUB) + c * SUB;
+      for (int k = 0; k < i; ++k) acc -= Lt[k * TSTR + i] * z[k];
+      z[i] = LDL ? acc : acc * ysh[SUB * Jt + i];
+    }
+    if (store) {
+      T* out = dinv + (int64_t)(t.c + Jt) * (SUB * SUB) + cz * SUB;
 #pragma unroll
       for (int i = 0; i < SUB; ++i) out[i] = z[i];
     }
   }
 
   // ---- write the factored lower triangle back (LDS reads first, one wait) ----
-  {
+  if (store) {
     const int r = tid & 15, c = tid >> 4;
     T v[NTL];
 #pragma unroll
@@ -430,6 +450,34 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
         if (row < bs && col < bs && row >= col) P[(int64_t)col * ld + row] = v[tslot(I, J)];
       }
   }
+
+  if (FUSED) {
+    // the TRSM reads the inverse tiles where the diagonal tiles were (and,
+    // for LDL, 1/D from ysh): every read of the diagonal tiles above ends
+    // before the first barrier, D is read before the second
+    __syncthreads();
+    if (LDL && tid < NBM * SUB) {
+      const int Jd = tid / SUB, jd = tid % SUB;
+      const T d = tl[tslot(Jd, Jd) * TSZ + jd * TSTR + jd];
+      ysh[tid] = (tid < bs) ? T(1) / d : T(0);
+    }
+    __syncthreads();
+    if (Jt < nb) {
+#pragma unroll
+      for (int i = 0; i < SUB; ++i) tl[tslot(Jt, Jt) * TSZ + cz * TSTR + i] = z[i];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool LDL, int NBM>
+__global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
+                                              const Task4* __restrict__ tasks,
+                                              T* __restrict__ arena, T* __restrict__ dinv,
+                                              int* __restrict__ info) {
+  __shared__ __attribute__((aligned(16))) DiagLds<T, NBM> L;
+  const Task4 t = tasks[blockIdx.x];
+  diag_body<T, LDL, NBM, false>(t, sn[t.a], arena, dinv, info, L, nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,6 +491,11 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
 // of tile J is k-chunk kk), so no data moves between the steps.
 // Task {sn, step, row_tile, inverse-tile offset}.
 // ---------------------------------------------------------------------------
+template <typename T, bool LDL, int NBM>
+__device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, int nb, int rt,
+                                              T* __restrict__ P, const T* __restrict__ tl,
+                                              const T* __restrict__ dd);
+
 template <typename T, bool LDL, int NBM>
 __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
                                                const Task4* __restrict__ tasks,
@@ -491,9 +544,21 @@ __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
     }
   }
   __syncthreads();
+  ptrsm_compute<T, LDL, NBM>(s, c0, bs, nb, t.c, P, tl, dd);
+}
 
+// The TRSM proper: rows of row tile rt below the block, with the strict
+// lower tiles and the inverse diagonal tiles in LDS (tl) and, for LDL, 1/D
+// of every column (dd).  P points at column c0 of the panel.
+template <typename T, bool LDL, int NBM>
+__device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, int nb, int rt,
+                                              T* __restrict__ P, const T* __restrict__ tl,
+                                              const T* __restrict__ dd) {
+  typedef typename Mfma<T>::v4 v4;
+  const int64_t ld = s.r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // rows of this wave: R0 + (lane & 15), columns as MFMA rows
-  const int R0 = c0 + bs + t.c * kTrsmRows + 16 * wave;
+  const int R0 = c0 + bs + rt * kTrsmRows + 16 * wave;
   if (R0 >= s.r) return;  // no barrier follows
   const int prow = min(R0 + (lane & 15), s.r - 1);
   const bool prow_ok = R0 + (lane & 15) < s.r;
@@ -549,6 +614,26 @@ __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
       }
     }
   }
+}
+
+// Block factor and panel TRSM in one workgroup per (supernode, row tile) of
+// a supernode of width <= kStep: every workgroup of a supernode factors the
+// diagonal block itself (used on etree levels with few supernodes, where the
+// GPU would otherwise idle between the two launches).  Task {sn, number of
+// row tiles, row tile, inverse-tile offset}; cnt: one zeroed counter per
+// supernode.
+template <typename T, bool LDL, int NBM>
+__global__ __launch_bounds__(NT) void k_diagtrsm(const SnInfo* __restrict__ sn,
+                                                  const Task4* __restrict__ tasks,
+                                                  T* __restrict__ arena, T* __restrict__ dinv,
+                                                  int* __restrict__ info, int* __restrict__ cnt) {
+  __shared__ __attribute__((aligned(16))) DiagLds<T, NBM> L;
+  const Task4 tk = tasks[blockIdx.x];
+  const Task4 t{tk.a, 0, tk.d, 0};  // the block factor's task layout {sn, step, inverse tile}
+  const SnInfo s = sn[tk.a];
+  diag_body<T, LDL, NBM, true>(t, s, arena, dinv, info, L, cnt + tk.a, tk.b);
+  const int bs = min(kStep, s.w);
+  ptrsm_compute<T, LDL, NBM>(s, 0, bs, (bs + SUB - 1) / SUB, tk.c, arena + s.off, L.tl, L.ysh);
 }
 
 // ---------------------------------------------------------------------------

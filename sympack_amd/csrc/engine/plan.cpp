@@ -223,6 +223,29 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     const size_t level_first = p.factor_seq.size();
     int smax = 0;
     for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kStep));
+    // Levels of at most kFuseMaxSn single-step supernodes: one fused
+    // block-factor + TRSM launch (every row-tile workgroup factors the
+    // block again) instead of two launches that each leave the GPU idle.
+    constexpr int kFuseMaxSn = 4;
+    if (smax == 1 && (int)sns.size() <= kFuseMaxSn) {
+      LaunchRange fr{9, (int64_t)p.fused_tasks.size(), 0};
+      int nbmax = 1;
+      for (Int I : sns) {
+        const SnInfo& si = p.sn[I];
+        const int bs = si.w;
+        nbmax = std::max(nbmax, (int)ceil_div(bs, kSub));
+        const int64_t rows_below = si.r - bs;
+        const int32_t inv_off = (int32_t)p.inv_base[I];
+        const int64_t nt = std::max<int64_t>(1, ceil_div(rows_below, kTrsmRows));
+        for (int64_t rt = 0; rt < nt; ++rt)
+          p.fused_tasks.push_back({I, (int32_t)nt, (int32_t)rt, inv_off});
+        p.diag_flops += (double)bs * bs * bs / 3.0 + (double)bs * bs * rows_below;
+      }
+      fr.aux = diag_bucket(nbmax);
+      fr.count = (int64_t)p.fused_tasks.size() - fr.first;
+      p.factor_seq.push_back(fr);
+      smax = 0;  // nothing left for the per-step launches below
+    }
     for (int st = 0; st < smax; ++st) {
       LaunchRange pr{0, (int64_t)p.potrf_tasks.size(), 0};
       LaunchRange tr{7, (int64_t)p.trsm_tasks.size(), 0};

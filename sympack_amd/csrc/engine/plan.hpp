@@ -59,7 +59,8 @@ struct MergeInfo {         // mirrored on the device
 };
 
 struct LaunchRange {
-  int kind;          // 0 diagonal factor, 7 panel TRSM, 1 in-panel update, 2 Schur update,
+  int kind;          // 0 diagonal factor, 7 panel TRSM, 9 both fused, 1 in-panel update,
+                     // 2 Schur update,
                      // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
@@ -88,6 +89,7 @@ struct Plan {
   int64_t inv_tiles = 0;           // 16x16 inverse diagonal tiles written by the factor
   std::vector<int64_t> inv_base;   // per supernode: first inverse tile (tile t of 16-column block t)
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
+  std::vector<Task4> fused_tasks;  // {sn, row tiles, row_tile, inverse-tile offset}: block factor + TRSM
   // {sn, ti, tj, d}: 64x64 output tiles; d = 0 plain, d > 0 the update also
   // carries merges[d-1] (a deferred child), d < 0 only output columns b < -d
   // (parent columns of a deferred supernode)
