@@ -183,7 +183,9 @@ void Engine::ensure_device() {
   d_info_ = dalloc<int>(1);
   if (!plan_.fused_tasks.empty()) {  // load counters of the fused block factor + TRSM
     d_fcnt_ = dalloc<int>((size_t)plan_.nsuper);
-    HIP_OK(hipMemset(d_fcnt_, 0, sizeof(int) * (size_t)plan_.nsuper));
+    // on the engine's stream: a legacy-stream memset would break another
+    // thread's graph capture
+    HIP_OK(hipMemsetAsync(d_fcnt_, 0, sizeof(int) * (size_t)plan_.nsuper, s));
   }
   if (plan_.df_ok) {
     d_df_fwd_ = dupload(plan_.df_fwd, s);

@@ -137,20 +137,16 @@ __device__ __forceinline__ float rcp_nr(float a) {
   y = y * (2.0f - a * y);
   return y;
 }
-// Reciprocal in three dependent operations after the hardware estimate:
-// with e = 1 - a y0, 1/a = y0 (1 + e + e^2 + ...), so y0 + y0 (e + e^2) leaves
-// a relative error of e^3.  The gfx950 f64 estimate has |e| < 2^-24 (measured
-// over 2^20 arguments spanning 2^-30..2^30), hence e^3 < 2^-72: full double
-// precision.  The f32 estimate is already within an ulp or two, one step.
-__device__ __forceinline__ double rcp_cubic(double a) {
-  const double y0 = __builtin_amdgcn_rcp(a);
-  const double e = fma(-a, y0, 1.0);
-  return fma(y0, fma(e, e, e), y0);
-}
-__device__ __forceinline__ float rcp_cubic(float a) {
-  const float y0 = __builtin_amdgcn_rcpf(a);
-  return fmaf(y0, fmaf(-a, y0, 1.0f), y0);
-}
+// Reciprocal from the hardware estimate y0 and the correction factor s:
+// with e = 1 - a y0, 1/a = y0 (1 + e + e^2 + ...), so y0 (1 + s) with
+// s = e + e^2 leaves a relative error of e^3.  The gfx950 f64 estimate has
+// |e| < 2^-24 (measured over 2^20 arguments spanning 2^-30..2^30), hence
+// e^3 < 2^-72: full double precision.  The f32 estimate is already within
+// an ulp or two: s = e, one step.
+__device__ __forceinline__ double rcp_est(double a) { return __builtin_amdgcn_rcp(a); }
+__device__ __forceinline__ float rcp_est(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double rcp_corr(double e) { return fma(e, e, e); }
+__device__ __forceinline__ float rcp_corr(float e) { return e; }
 
 __device__ __forceinline__ double readlane_t(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -213,7 +209,7 @@ struct DiagLds {
   T tl[NBM * (NBM + 1) / 2 * TSZ];
   T dsh[SUB];
   T ysh[NBM * SUB];  // LL: 1/L_jj of every column; LDL (fused): 1/D_j
-  T colb[NT / 64][SUB];
+  T colb[NT / 64][2][SUB];  // per wave: the current diagonal-row column, double-buffered
   int store;  // fused: this workgroup writes the factored block back
 };
 
@@ -232,7 +228,7 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
   T* __restrict__ tl = L.tl;
   T* __restrict__ dsh = L.dsh;
   T* __restrict__ ysh = L.ysh;
-  T (*colb)[SUB] = L.colb;
+  T (*colb)[2][SUB] = L.colb;
   const int c0 = t.b * kStep;
   const int bs = min(kStep, s.w - c0);
   const int nb = (bs + SUB - 1) / SUB;
@@ -293,19 +289,8 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
   const bool diag_lane = lane < SUB;
   const int rb = 48 * wave + lane - SUB;  // row below the diagonal tile (lanes >= 16)
   for (int J = 0; J < nb; ++J) {
-    // ---- 0. the tiles of block column J leave the accumulators ----
-    if (J > 0) {
-#pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        if (tK[q] == J) {
-          T* C = tl + tslot(tI[q], J) * TSZ;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[q][v];
-        }
-      }
-      __syncthreads();
-    }
-    // ---- 1. block column J, lane per row ----
+    // ---- 1. block column J, lane per row (its tiles were written to LDS
+    // by the initial load or at the end of step J-1) ----
     const int Ib = J + 1 + (diag_lane ? 0 : rb / SUB);
     const bool row_ok = diag_lane || (Ib < nb);
     const int I = diag_lane ? J : min(Ib, nb - 1);
@@ -314,68 +299,78 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
     T a[SUB];
 #pragma unroll
     for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
-    // every wave reads the diagonal tile, which wave 0 overwrites below
-    __syncthreads();
-    T ylane = T(1);  // LL: 1/L_jj of this lane's diagonal row
-    T dlane = T(1);  // pivot d_j of diagonal row j (lane j)
-    int first_bad = INT_MAX;  // first failing pivot column of this block column (uniform)
-    // column j of the diagonal rows (still unscaled) is broadcast through
+    // every wave reads the diagonal tile: wave 0 writes the factored one
+    // back only after the barrier that ends this phase
+    // Column j of the diagonal rows (still unscaled) is broadcast through
     // this wave's LDS buffer: LDS operations of one wave complete in order,
-    // so no barrier is needed between the write and the reads
-    T* __restrict__ cb = colb[wave];
+    // so no barrier is needed between the write and the reads.  Column j+1
+    // is final on the diagonal rows as soon as step j has updated it (the
+    // first thing step j does), so it is published and read back in step j,
+    // a step ahead of its use: the LDS round trip stays off the wave's path.
+    T (*__restrict__ cb)[SUB] = colb[wave];
+    T u[SUB];  // u[k], k >= j+3: column j of diagonal row k
+    if (diag_lane) cb[0][lane] = a[0];
+    compiler_fence();
+#pragma unroll
+    for (int k = 3; k < SUB; ++k) u[k] = cb[0][k];
+    compiler_fence();
 #pragma unroll
     for (int j = 0; j < SUB; ++j) {
       // The dependency chain from one pivot to the next runs through
       // readlane(pivot) -> 1/pivot -> the next row's diagonal entry; the
-      // next row's entry comes by readlane, the rest of the column through
-      // LDS, and the scaling 1/sqrt(pivot) stays off the chain.
-      if (diag_lane) cb[lane] = a[j];
-      compiler_fence();
+      // next row's entry comes by readlane, and the scaling 1/sqrt(pivot)
+      // stays off the chain.
       const T piv = readlane_t(a[j], j);
-      T u[SUB];
-      // u[j+1] and u[j+2] feed the next two pivots: readlane; the rest
-      // through LDS, whose latency then has two steps of slack
+      // u[j+1] and u[j+2] feed the next two pivots: readlane
       if (j + 1 < SUB) u[j + 1] = readlane_t(a[j], j + 1);
       if (j + 2 < SUB) u[j + 2] = readlane_t(a[j], j + 2);
+      // 1/piv from the hardware estimate y0 and e = 1 - piv y0:
+      // inv = y0 (1 + s) with s = e + e^2 (f64; |e| < 2^-24, so the error is
+      // e^3) or s = e (f32)
+      const T y0 = rcp_est(piv);
+      const T e = fma(-piv, y0, T(1));
+      const T s = rcp_corr(e);
+      const T inv = fma(y0, s, y0);
+      // a_k -= (a_j / piv) u_k.  The entry that feeds the next pivot is
+      // a - t (1 + s) with t = a_j u_{j+1} y0: t and e are formed side by
+      // side, so it is three dependent operations after the estimate
+      if (j + 1 < SUB) {
+        const T t = (a[j] * u[j + 1]) * y0;
+        a[j + 1] = fma(-t, s, a[j + 1] - t);
+      }
+      // column j+1 of the diagonal rows is final: publish it, read the next
+      // step's entries (k >= j+4)
+      T un[SUB];
+      if (j + 1 < SUB) {
+        if (diag_lane) cb[(j + 1) & 1][lane] = a[j + 1];
+        compiler_fence();
 #pragma unroll
-      for (int k = j + 3; k < SUB; ++k) u[k] = cb[k];
-      compiler_fence();
-      // a failing pivot is recorded off the chain; the factor is then
-      // invalid anyway, the values that follow are not used
-      const bool bad = LDL ? (piv == T(0) || piv != piv) : !(piv > T(0));
-      if (bad && SUB * J + j < bs) first_bad = min(first_bad, SUB * J + j);
-      const T inv = rcp_cubic(piv);
-      if (lane == j) dlane = piv;
-      // a_k -= (a_j / piv) u_k; the entry that feeds the next pivot is
-      // formed as (a_j u_{j+1}) * inv, one operation after inv
-      if (j + 1 < SUB) a[j + 1] = fma(-(a[j] * u[j + 1]), inv, a[j + 1]);
+        for (int k = j + 4; k < SUB; ++k) un[k] = cb[(j + 1) & 1][k];
+        compiler_fence();
+      }
       const T l = a[j] * inv;
 #pragma unroll
       for (int k = j + 2; k < SUB; ++k) a[k] -= l * u[k];
-      // LL keeps the unscaled column (a_ij) for now; LDL stores l = a_ij / d_j
+      // LL: L_ij = a_ij / sqrt(d_j), with 1/sqrt(d_j) formed from the
+      // uniform pivot off the chain; LDL stores l = a_ij / d_j and d_j
       if (LDL) a[j] = (lane == j) ? piv : l;
-    }
-    if (!LDL) {
-      // L_ij = a_ij / sqrt(d_j): lane j computes 1/sqrt(d_j) for its
-      // diagonal row, the 16 factors are broadcast by readlane
-      ylane = rsqrt_nr(dlane);
+      else a[j] = a[j] * rsqrt_nr(piv);
 #pragma unroll
-      for (int j = 0; j < SUB; ++j) {
-        const T y = readlane_t(ylane, j);
-        a[j] = a[j] * y;
-      }
+      for (int k = j + 4; k < SUB; ++k) u[k] = un[k];
     }
-    // diagonal tile: lower part only (wave 0); rows below: every valid lane
-    if ((diag_lane && wave == 0) || (!diag_lane && row_ok)) {
+    // rows below: every valid lane (the trailing update reads them)
+    if (!diag_lane && row_ok) {
 #pragma unroll
-      for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (diag_lane && c > lane) ? T(0) : a[c];
+      for (int c = 0; c < SUB; ++c) trow[c * TSTR] = a[c];
     }
-    if (wave == 0 && diag_lane) {
-      if (LDL) dsh[lane] = a[lane];
-      else ysh[SUB * J + lane] = ylane;
-    }
-    if (first_bad != INT_MAX && wave == 0 && lane == 0) atomicMin(info, s.fc + c0 + first_bad + 1);
+    if (LDL && wave == 0 && diag_lane) dsh[lane] = a[lane];
     __syncthreads();
+    // every wave has read the unfactored diagonal tile: wave 0 stores the
+    // factored lower part (read again only after the loop)
+    if (diag_lane && wave == 0) {
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (c > lane) ? T(0) : a[c];
+    }
     // ---- 2. trailing update A_IK -= L_IJ L_KJ^T of the register tiles with
     // K > J (a prefix of each wave's slots); operands of four slots are
     // loaded together, the MFMAs of the active ones issue back to back ----
@@ -408,8 +403,30 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
           }
       }
     }
+    // ---- 3. the tiles of block column J+1 are final: they leave the
+    // accumulators (step 2 of other waves reads only column J) ----
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (J + 1 < nb && tK[q] == J + 1) {  // unused slots hold tK = NBM
+        T* C = tl + tslot(tI[q], J + 1) * TSZ;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C[(lane & 15) * TSTR + Mfma<T>::row(lane, v)] = acc[q][v];
+      }
+    }
     __syncthreads();
   }
+
+  // ---- failed pivots (a pivot that is not > 0, LL, or is 0 or NaN, LDL;
+  // NaN spreads to every later pivot, so the smallest column is the first
+  // failure) and, for LL, 1/L_jj for the inverse tiles ----
+  if (tid < nb * SUB) {
+    const int Jd = tid / SUB, jd = tid % SUB;
+    const T dj = tl[tslot(Jd, Jd) * TSZ + jd * TSTR + jd];
+    const bool bad = LDL ? (dj == T(0) || dj != dj) : !(dj > T(0));
+    if (bad && tid < bs) atomicMin(info, s.fc + c0 + tid + 1);
+    if (!LDL) ysh[tid] = T(1) / dj;
+  }
+  __syncthreads();
 
   // ---- inverses of the factored 16x16 diagonal tiles for the panel TRSM,
   // all at once: 16 lanes per tile, lane (c) solves L z = e_c (LDL: unit

@@ -223,11 +223,14 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     const size_t level_first = p.factor_seq.size();
     int smax = 0;
     for (Int I : sns) smax = std::max<int>(smax, (int)ceil_div(p.sn[I].w, kStep));
-    // Levels of at most kFuseMaxSn single-step supernodes: one fused
-    // block-factor + TRSM launch (every row-tile workgroup factors the
-    // block again) instead of two launches that each leave the GPU idle.
-    constexpr int kFuseMaxSn = 4;
-    if (smax == 1 && (int)sns.size() <= kFuseMaxSn) {
+    // Levels of single-step supernodes with at most kFuseMaxTasks row
+    // tiles in all: one fused block-factor + TRSM launch (every row-tile
+    // workgroup factors the block again) instead of two dependent launches.
+    constexpr int64_t kFuseMaxTasks = 256;
+    int64_t fuse_tasks = 0;
+    if (smax == 1)
+      for (Int I : sns) fuse_tasks += std::max<int64_t>(1, ceil_div(p.sn[I].r - p.sn[I].w, kTrsmRows));
+    if (smax == 1 && fuse_tasks <= kFuseMaxTasks) {
       LaunchRange fr{9, (int64_t)p.fused_tasks.size(), 0};
       int nbmax = 1;
       for (Int I : sns) {
