@@ -203,6 +203,24 @@ __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 +
 // {sn, step, inverse-tile offset, -}; info receives the smallest failing
 // pivot column (1-based, permuted order).
 // ---------------------------------------------------------------------------
+// Slots 0..na-1 of a wave's trailing tiles as nested conditionals (slot q+1
+// is reached only from slot q): the operands of slot q+1 are loaded before
+// the MFMAs of slot q, and the compiler's LDS wait counts stay exact.
+template <int Q, int NS>
+struct SlotChain {
+  template <class Load, class Mma>
+  static __device__ __forceinline__ void run(int na, Load& load, Mma& mma) {
+    if constexpr (Q < NS) {
+      if (Q < na) {
+        if constexpr (Q + 1 < NS) load(Q + 1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the MFMAs
+        mma(Q);
+        SlotChain<Q + 1, NS>::run(na, load, mma);
+      }
+    }
+  }
+};
+
 // LDS of the block factor (and of the fused block factor + TRSM)
 template <typename T, int NBM>
 struct DiagLds {
@@ -372,37 +390,37 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
       for (int c = 0; c < SUB; ++c) trow[c * TSTR] = (c > lane) ? T(0) : a[c];
     }
     // ---- 2. trailing update A_IK -= L_IJ L_KJ^T of the register tiles with
-    // K > J (a prefix of each wave's slots); operands of four slots are
-    // loaded together, the MFMAs of the active ones issue back to back ----
+    // K > J (a prefix of each wave's slots).  The operands of slot q+1 are
+    // read from LDS before the four MFMAs of slot q issue, so their latency
+    // hides behind the MFMA pipe ----
     const int m = nb - J - 1;
     const int ntr = m * (m + 1) / 2;
     const int na = (ntr - wave + 3) >> 2;  // active slots of this wave
+    T dk[4];  // LDL: d_k of this lane's four k rows
 #pragma unroll
-    for (int g = 0; g < NS; g += 4) {
-      if (g < na) {
-        T av[4][4], bv[4][4];
+    for (int kk = 0; kk < 4; ++kk) dk[kk] = LDL ? dsh[Mfma<T>::row(lane, kk)] : T(1);
+    // Loads are unconditional (an inactive slot reads some valid tile) and a
+    // slot is reached only through the one before it, so the LDS wait
+    // before slot q's MFMAs covers exactly slot q's operands.
+    T av[2][4], bv[2][4];
+    auto load_slot = [&](int q, T* ao, T* bo) {
+      const T* A = tl + tslot(min(tI[q], nb - 1), J) * TSZ + (lane & 15);
+      const T* B = tl + tslot(min(tK[q], nb - 1), J) * TSZ + (lane & 15);
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          const int q = min(g + h, NS - 1);
-          const bool on = g + h < na;
-          const T* A = tl + tslot(on ? tI[q] : J, J) * TSZ;
-          const T* B = tl + tslot(on ? tK[q] : J, J) * TSZ;
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) {
-            const int kp = Mfma<T>::row(lane, kk);
-            av[h][kk] = A[kp * TSTR + (lane & 15)] * (on ? T(1) : T(0));
-            if (LDL) av[h][kk] *= dsh[kp];
-            bv[h][kk] = B[kp * TSTR + (lane & 15)];
-          }
-        }
-#pragma unroll
-        for (int h = 0; h < 4; ++h)
-          if (g + h < NS && g + h < na) {
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) acc[g + h] = Mfma<T>::mma(-av[h][kk], bv[h][kk], acc[g + h]);
-          }
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        ao[kk] = A[kp * TSTR];
+        bo[kk] = B[kp * TSTR];
       }
-    }
+    };
+    auto load = [&](int q) { load_slot(q, av[q & 1], bv[q & 1]); };
+    auto mma = [&](int q) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        acc[q] = Mfma<T>::mma(LDL ? -(av[q & 1][kk] * dk[kk]) : -av[q & 1][kk], bv[q & 1][kk], acc[q]);
+    };
+    load(0);
+    SlotChain<0, NS>::run(na, load, mma);
     // ---- 3. the tiles of block column J+1 are final: they leave the
     // accumulators (step 2 of other waves reads only column J) ----
 #pragma unroll
