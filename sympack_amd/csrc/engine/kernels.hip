@@ -759,19 +759,29 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   }
   const int nq1 = (K + KC - 1) / KC;
   const int nq = nq1 + (K2 + KC - 1) / KC;
-  T ra[NQ], rb[NQ];
+  // A chunk is loaded from clamped (valid) addresses into ra/rb with its
+  // validity bits in mk; the bits are applied when the chunk is written to
+  // LDS, a chunk later, so nothing consumes the loads before the MFMAs of
+  // the current chunk have issued.  LDL: dv holds d_k of k column tid.
+  T ra[NQ], rb[NQ], dv = T(0);
+  unsigned mk = 0;
   auto fetch = [&](int qc) {
+    unsigned m = 0;
     if (qc < nq1) {
       const int kc = qc * KC;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int kg = kc + lk + 4 * q;
         const T* col = P + (int64_t)(kbeg + min(kg, K - 1)) * ld;
-        // clamped addresses hold finite values: masking by multiplication
-        // keeps the loads unconditional (a select would be turned into a
-        // predicated load with its own wait)
-        ra[q] = col[arow] * T((kg < K) & arow_ok);
-        rb[q] = col[brow] * T((kg < K) & brow_ok);
+        ra[q] = col[arow];
+        rb[q] = col[brow];
+        m |= (unsigned)((kg < K) & arow_ok) << q;
+        m |= (unsigned)((kg < K) & brow_ok) << (q + NQ);
+      }
+      if (LDL) {
+        const int kg = min(kc + (tid & (KC - 1)), K - 1);
+        dv = P[(int64_t)(kbeg + kg) * ld + kbeg + kg];
+        m |= (unsigned)(kc + (tid & (KC - 1)) < K) << (2 * NQ);
       }
     } else {
       const int kc = (qc - nq1) * KC;
@@ -779,10 +789,18 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
       for (int q = 0; q < NQ; ++q) {
         const int kg = kc + lk + 4 * q;
         const T* col = P2 + (int64_t)min(kg, K2 - 1) * ld2;
-        ra[q] = col[arow2] * T((kg < K2) & arow2_ok);
-        rb[q] = col[brow2] * T((kg < K2) & brow2_ok);
+        ra[q] = col[arow2];
+        rb[q] = col[brow2];
+        m |= (unsigned)((kg < K2) & arow2_ok) << q;
+        m |= (unsigned)((kg < K2) & brow2_ok) << (q + NQ);
+      }
+      if (LDL) {
+        const int kg = min(kc + (tid & (KC - 1)), K2 - 1);
+        dv = P2[(int64_t)kg * ld2 + kg];
+        m |= (unsigned)(kc + (tid & (KC - 1)) < K2) << (2 * NQ);
       }
     }
+    mk = m;
   };
   fetch(0);
   // MODE 1: scatter offsets of the 16 outputs of this lane, loaded while
@@ -817,18 +835,10 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   for (int qc = 0; qc < nq; ++qc) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      As[lk + 4 * q][lrow] = ra[q];
-      Bs[lk + 4 * q][lrow] = rb[q];
+      As[lk + 4 * q][lrow] = ((mk >> q) & 1u) ? ra[q] : T(0);
+      Bs[lk + 4 * q][lrow] = ((mk >> (q + NQ)) & 1u) ? rb[q] : T(0);
     }
-    if (LDL && tid < KC) {
-      if (qc < nq1) {
-        const int kg = qc * KC + tid;
-        Dk[tid] = (kg < K) ? P[(int64_t)(kbeg + kg) * ld + kbeg + kg] : T(0);
-      } else {
-        const int kg = (qc - nq1) * KC + tid;
-        Dk[tid] = (kg < K2) ? P2[(int64_t)kg * ld2 + kg] : T(0);
-      }
-    }
+    if (LDL && tid < KC) Dk[tid] = ((mk >> (2 * NQ)) & 1u) ? dv : T(0);
     __syncthreads();
     if (qc + 1 < nq) fetch(qc + 1);  // next chunk in flight during the MFMAs
 #pragma unroll
