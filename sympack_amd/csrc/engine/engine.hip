@@ -530,17 +530,19 @@ void Engine::factor_sequence(T* arena) {
       case 1: {
         const Task4* t = (const Task4*)d_tasks_[TA_INPANEL] + r.first;
         if (ldl)
-          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
+          dev::k_update<T, 0, true><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, 1);
         else
-          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
+          dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, 1);
         break;
       }
       case 2: {
         const Task4* t = (const Task4*)d_tasks_[TA_SCHUR] + r.first;
+        const int ks = std::max(1, r.aux);  // k-chunk split: one workgroup per (task, part)
+        const unsigned gs = g * (unsigned)ks;
         if (ldl)
-          dev::k_update<T, 1, true><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
+          dev::k_update<T, 1, true><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, ks);
         else
-          dev::k_update<T, 1, false><<<g, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count);
+          dev::k_update<T, 1, false><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, ks);
         break;
       }
       default:

@@ -689,15 +689,19 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
                                                 const int64_t* __restrict__ tmap,
                                                 const int32_t* __restrict__ rel,
                                                 const MergeInfo* __restrict__ merges,
-                                                const int32_t* __restrict__ mmap, int64_t ntasks) {
+                                                const int32_t* __restrict__ mmap, int64_t ntasks,
+                                                int ksplit) {
   typedef typename Mfma<T>::v4 v4;
-  constexpr int KC = 32;
+  constexpr int KC = kSchurKC;
   constexpr int LDS_LD = TILE + 1;  // odd pitch: the 4 k rows a wave reads hit different banks
   __shared__ T As[KC][LDS_LD];
   __shared__ T Bs[KC][LDS_LD];
   __shared__ T Dk[KC];
-  // grid-stride over the tasks (the launches use one workgroup per task)
-  for (int64_t task = blockIdx.x; task < ntasks; task += gridDim.x) {
+  // grid-stride over (task, k part) pairs (the launches use one workgroup
+  // per pair); part `part` of a task covers k chunks [q_lo, q_hi)
+  for (int64_t gi = blockIdx.x; gi < ntasks * ksplit; gi += gridDim.x) {
+  const int64_t task = gi / ksplit;
+  const int part = (int)(gi - task * ksplit);
   const Task4 t = tasks[task];
   const SnInfo s = sn[t.a];
   T* __restrict__ P = arena + s.off;
@@ -759,6 +763,8 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   }
   const int nq1 = (K + KC - 1) / KC;
   const int nq = nq1 + (K2 + KC - 1) / KC;
+  const int q_lo = (int)((int64_t)part * nq / ksplit), q_hi = (int)((int64_t)(part + 1) * nq / ksplit);
+  if (q_lo >= q_hi) continue;  // uniform over the workgroup
   // A chunk is loaded from clamped (valid) addresses into ra/rb with its
   // validity bits in mk; the bits are applied when the chunk is written to
   // LDS, a chunk later, so nothing consumes the loads before the MFMAs of
@@ -802,7 +808,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     }
     mk = m;
   };
-  fetch(0);
+  fetch(q_lo);
   // MODE 1: scatter offsets of the 16 outputs of this lane, loaded while
   // the first operand chunk is in flight (rel is defined for rows a >= the
   // first row of b's target group: clamped, unconditional loads)
@@ -832,7 +838,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         }
       }
   }
-  for (int qc = 0; qc < nq; ++qc) {
+  for (int qc = q_lo; qc < q_hi; ++qc) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       As[lk + 4 * q][lrow] = ((mk >> q) & 1u) ? ra[q] : T(0);
@@ -840,7 +846,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     }
     if (LDL && tid < KC) Dk[tid] = ((mk >> (2 * NQ)) & 1u) ? dv : T(0);
     __syncthreads();
-    if (qc + 1 < nq) fetch(qc + 1);  // next chunk in flight during the MFMAs
+    if (qc + 1 < q_hi) fetch(qc + 1);  // next chunk in flight during the MFMAs
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
       const int k = kk * 4 + (lane >> 4);

@@ -2,6 +2,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 
 namespace spk {
@@ -312,6 +313,19 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       p.schur_flops += (double)si.w * (double)m * (double)(m + 1);
     }
     sr.count = (int64_t)p.schur_tasks.size() - sr.first;
+    if (sr.count && sr.count < kSplitTarget) {
+      // few output tiles: each task's k chunks are split over S workgroups
+      // (partial sums meet in the atomic scatter), S <= the fewest chunks of
+      // any task of the level so that no part is empty
+      int nqmin = INT_MAX;
+      for (int64_t q = sr.first; q < sr.first + sr.count; ++q) {
+        const Task4& t = p.schur_tasks[q];
+        int nq = (int)ceil_div(p.sn[t.a].w, kSchurKC);
+        if (t.d > 0) nq += (int)ceil_div(p.merges[t.d - 1].w, kSchurKC);
+        nqmin = std::min(nqmin, nq);
+      }
+      sr.aux = (int)std::max<int64_t>(1, std::min<int64_t>(nqmin, kSplitTarget / sr.count));
+    }
     if (sr.count) p.factor_seq.push_back(sr);
     for (size_t q = level_first; q < p.factor_seq.size(); ++q) p.factor_seq[q].level = L;
   }

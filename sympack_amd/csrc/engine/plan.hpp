@@ -25,6 +25,11 @@ constexpr int kTile = 64;    // update tile edge (rows x cols)
 constexpr int kStep = 160;   // panel columns factored per step (diagonal block <= 160 x 160)
 constexpr int kSub = 16;     // sub-block edge inside a diagonal block (MFMA 16x16 tiles)
 constexpr int kTrsmRows = 64;  // panel rows per TRSM workgroup (16 per wave)
+constexpr int kSchurKC = 32;   // k columns per LDS chunk of the Schur update
+// Schur launches with fewer tasks than this (one per CU of the 256) split
+// each task's k chunks over several workgroups; more splitting costs more in
+// extra scatter than it gains in latency
+constexpr int64_t kSplitTarget = 256;
 
 // LDS bucket of a diagonal block with nb 16-column sub-blocks: kernels are
 // compiled for nb <= 1, 2, 4, 6, 10.
@@ -64,7 +69,8 @@ struct LaunchRange {
                      // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
-  int aux = 0;       // kinds 0, 7, 3, 6: diagonal-block bucket (diag_bucket)
+  int aux = 0;       // kinds 0, 7, 9, 3, 6: diagonal-block bucket (diag_bucket);
+                     // kind 2: k-chunk split factor (0 or 1 = none)
   int level = 0;     // etree level of the launch (factor sequence)
 };
 
