@@ -199,7 +199,7 @@ def test_graph_replays_match_eager(prec):
     E.reset_kernel_stats()
     G.Factorize()
     E.Factorize()
-    for k in range(8):
+    for k in range(10):
         assert G.kernel_stats(k)[0] == E.kernel_stats(k)[0]
 
 
