@@ -809,35 +809,6 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     mk = m;
   };
   fetch(q_lo);
-  // MODE 1: scatter offsets of the 16 outputs of this lane, loaded while
-  // the first operand chunk is in flight (rel is defined for rows a >= the
-  // first row of b's target group: clamped, unconditional loads)
-  int64_t off[2][4][2];
-  if (MODE == 1) {
-    const int m = s.r - s.w;
-    int64_t tm[2][4];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
-        off[bi][v][0] = tcol[s.upd + bc];
-        tm[bi][v] = tmap[s.upd + bc];
-      }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
-        const int64_t tc = off[bi][v][0];
-#pragma unroll
-        for (int aj = 0; aj < 2; ++aj) {
-          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
-          const int ac = min(max(aoff, bc), m - 1);
-          off[bi][v][aj] = tc + rel[tm[bi][v] + ac];
-        }
-      }
-  }
   for (int qc = q_lo; qc < q_hi; ++qc) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -866,6 +837,37 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     __syncthreads();
   }
 
+  // MODE 1: scatter offsets of the 16 outputs of this lane (rel is defined
+  // for rows a >= the first row of b's target group: clamped, unconditional
+  // loads).  Loaded after the k loop: out of the loop's registers (124
+  // VGPRs) four workgroups share a CU, which hides their latency better
+  // than loading them with the first chunk (148 VGPRs, three workgroups).
+  int64_t off[2][4][2];
+  if (MODE == 1) {
+    const int m = s.r - s.w;
+    int64_t tm[2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
+        off[bi][v][0] = tcol[s.upd + bc];
+        tm[bi][v] = tmap[s.upd + bc];
+      }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
+        const int64_t tc = off[bi][v][0];
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) {
+          const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
+          const int ac = min(max(aoff, bc), m - 1);
+          off[bi][v][aj] = tc + rel[tm[bi][v] + ac];
+        }
+      }
+  }
   // Epilogue: every load first (clamped, unconditional, one wait), then the
   // predicated stores / atomics.
   if (MODE == 0) {
