@@ -287,7 +287,7 @@ def main():
         sol_ms.append((d - c) * 1e3)
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
-    launches = {k: M.kernel_stats(k)[0] for k in range(8)}
+    launches = {k: M.kernel_stats(k)[0] for k in range(10)}
 
     # timed region 2: K factorizations launched from the host with HIP events
     # around every Schur-update launch (kind 2) on the library's stream, for
@@ -387,7 +387,8 @@ def main():
         "backward_error": berr,
         "factor_ms_eager_profiled": round(float(np.mean(prof_ms)), 3),
         "launches_per_step": {name: launches[k] / args.steps
-                              for k, name in [(0, "potrf"), (7, "trsm"), (1, "inpanel_update"),
+                              for k, name in [(0, "potrf"), (7, "trsm"), (9, "potrf_trsm_fused"),
+                                              (1, "inpanel_update"),
                                               (2, "schur_update"), (3, "fwd_diag"), (4, "fwd_offdiag"),
                                               (5, "bwd_offdiag"), (6, "bwd_diag")]},
         "roofline": {"kernel": "k_update<double,1> (Schur update, FP64 MFMA + atomic scatter)",
