@@ -13,13 +13,14 @@
 //                LDS (blockCell_t::factorize, symPACKMatrix2D.hpp:1052-1102).
 //  k_ptrsm       panel TRSM of 64 rows below that block on MFMA with the
 //                inverted 16x16 diagonal tiles (blockCell_t::trsm, :1104-1178).
+//  k_diagtrsm    both, one workgroup per 64-row tile (each factors the block).
 //  k_update<0>   trailing update of a panel wider than 160 inside itself.
 //  k_update<1>   Schur update of ancestors: -P P^T (LDL: -P D P^T) on FP64 /
 //                FP32 MFMA 16x16x4, scattered straight from registers into the
 //                target panels through precomputed relative indices with
 //                hardware atomics (blockCell_t::update, :1180-1466).
-//  k_sdiag3 / k_sdiag_fwd / k_soffm_fwd / k_scale_d / k_soffm_bwd / k_sdiag_bwd
-//                supernodal forward/backward solve (forward_update[_contrib],
+//  k_sdiag4 / k_sdiag3 / k_sdiag_fwd / k_soffm_fwd / k_scale_d / k_soffm_bwd /
+//  k_sdiag_bwd   supernodal forward/backward solve (forward_update[_contrib],
 //                back_update[_contrib], :1486-1797).
 #include <hip/hip_runtime.h>
 
@@ -196,7 +197,8 @@ __host__ __device__ constexpr int tslot(int I, int J) { return I * (I + 1) / 2 +
 //     pivot columns are broadcast through a per-wave LDS buffer without
 //     barriers), lanes 16-63 hold 48 rows below it;
 //  2. the trailing lower tiles get A_IK -= L_IJ L_KJ^T (LDL: L_IJ D_J L_KJ^T)
-//     on MFMA 16x16x4, four independent tiles per wave at a time.
+//     on MFMA 16x16x4, each wave's tiles one after another with the next
+//     tile's operands read from LDS ahead of the current tile's MFMAs.
 // Finally the 16x16 diagonal tiles are inverted (for the panel TRSM), all in
 // one pass with 16 lanes per tile.
 // Rows/columns beyond bs are padded with an identity block.  Task
