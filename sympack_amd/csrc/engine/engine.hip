@@ -50,6 +50,133 @@ inline unsigned grid_for(int64_t n, int block = 256) {
 enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED };
 }  // namespace
 
+namespace {
+// Per-column reductions of column-major n x nrhs vectors in two passes with
+// a fixed summation order: kRedBlocks blocks per column write partial
+// results, one block per column combines them.
+__device__ __forceinline__ void block_reduce2(double& a, double& b, bool is_max) {
+  __shared__ double sa[256], sb[256];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      if (is_max) {
+        sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+        sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + s]);
+      } else {
+        sa[threadIdx.x] += sa[threadIdx.x + s];
+      }
+    }
+    __syncthreads();
+  }
+  a = sa[0];
+  b = sb[0];
+  __syncthreads();
+}
+
+// partial max |r| and max |x| of column blockIdx.y over block blockIdx.x's rows
+__global__ void k_colmax2_part(const double* __restrict__ r, const double* __restrict__ x, int n,
+                               double* __restrict__ part) {
+  const int j = blockIdx.y;
+  const int64_t per = ((int64_t)n + kRedBlocks - 1) / kRedBlocks;
+  const int64_t i0 = (int64_t)blockIdx.x * per, i1 = std::min<int64_t>(n, i0 + per);
+  double mr = 0.0, mx = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    mr = fmax(mr, fabs(r[(int64_t)j * n + i]));
+    mx = fmax(mx, fabs(x[(int64_t)j * n + i]));
+  }
+  block_reduce2(mr, mx, true);
+  if (threadIdx.x == 0) {
+    part[2 * ((int64_t)j * kRedBlocks + blockIdx.x)] = mr;
+    part[2 * ((int64_t)j * kRedBlocks + blockIdx.x) + 1] = mx;
+  }
+}
+
+// out[2j] = max |r_j|, out[2j+1] = max |x_j|
+__global__ void k_colmax2_fin(const double* __restrict__ part, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const int t = threadIdx.x;
+  double mr = t < kRedBlocks ? part[2 * ((int64_t)j * kRedBlocks + t)] : 0.0;
+  double mx = t < kRedBlocks ? part[2 * ((int64_t)j * kRedBlocks + t) + 1] : 0.0;
+  block_reduce2(mr, mx, true);
+  if (t == 0) {
+    out[2 * j] = mr;
+    out[2 * j + 1] = mx;
+  }
+}
+
+// partial sums of a_j . b_j
+__global__ void k_coldot_part(const double* __restrict__ a, const double* __restrict__ b, int n,
+                              double* __restrict__ part) {
+  const int j = blockIdx.y;
+  const int64_t per = ((int64_t)n + kRedBlocks - 1) / kRedBlocks;
+  const int64_t i0 = (int64_t)blockIdx.x * per, i1 = std::min<int64_t>(n, i0 + per);
+  double acc = 0.0, unused = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += a[(int64_t)j * n + i] * b[(int64_t)j * n + i];
+  block_reduce2(acc, unused, false);
+  if (threadIdx.x == 0) part[(int64_t)j * kRedBlocks + blockIdx.x] = acc;
+}
+
+__global__ void k_coldot_fin(const double* __restrict__ part, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const int t = threadIdx.x;
+  double acc = t < kRedBlocks ? part[(int64_t)j * kRedBlocks + t] : 0.0, unused = 0.0;
+  block_reduce2(acc, unused, false);
+  if (t == 0) out[j] = acc;
+}
+
+// CSR values from the caller's CSC values
+__global__ void k_gather_vals(double* __restrict__ cval, const double* __restrict__ vals,
+                              const int64_t* __restrict__ src, int64_t nnz) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) cval[k] = vals[src[k]];
+}
+
+// out = b - A x (b given) or out = A x, A symmetric as full CSR, vectors
+// column-major n x nrhs.  One wave per row: lane (e, j) = (lane / 16,
+// lane % 16) sums entries e, e + 4, ... of the row for right-hand side j of
+// each group of 16; no atomics, a fixed summation order.
+__global__ void k_csr_mv(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                         const double* __restrict__ val, const double* __restrict__ x,
+                         const double* __restrict__ b, double* __restrict__ out, int n, int nrhs) {
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int lane = threadIdx.x & 63, e = lane >> 4, j16 = lane & 15;
+  const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+  for (int j0 = 0; j0 < nrhs; j0 += 16) {
+    const int j = j0 + j16;
+    const double* __restrict__ xj = x + (int64_t)std::min(j, nrhs - 1) * n;
+    double acc = 0.0;
+    for (int64_t p = p0 + e; p < p1; p += 4) acc += val[p] * xj[col[p]];
+    acc += __shfl_xor(acc, 16);
+    acc += __shfl_xor(acc, 32);
+    if (e == 0 && j < nrhs) {
+      const int64_t o = (int64_t)j * n + row;
+      out[o] = b ? b[o] - acc : acc;
+    }
+  }
+}
+}  // namespace
+
+// per-RHS max |r| and max |x| into out[2j], out[2j+1]
+void Engine::colmax2(const double* r, const double* x, int nrhs, double* out) {
+  k_colmax2_part<<<dim3(kRedBlocks, (unsigned)nrhs), 256, 0, stream_>>>(r, x, n_, d_part_);
+  k_colmax2_fin<<<(unsigned)nrhs, 256, 0, stream_>>>(d_part_, out);
+}
+
+// out[j] = a_j . b_j
+void Engine::coldot(const double* a, const double* b, int nrhs, double* out) {
+  k_coldot_part<<<dim3(kRedBlocks, (unsigned)nrhs), 256, 0, stream_>>>(a, b, n_, d_part_);
+  k_coldot_fin<<<(unsigned)nrhs, 256, 0, stream_>>>(d_part_, out);
+}
+
+// y = A x on the full CSR (FP64, original numbering)
+void Engine::csr_mv(const double* x, const double* b, double* out, int nrhs) {
+  const unsigned g = (unsigned)((n_ + 3) / 4);
+  k_csr_mv<<<g, 256, 0, stream_>>>(d_crow_, d_ccol_, d_cval_, x, b, out, n_, nrhs);
+}
+
 Engine::Engine() { kind_stats.resize(10); }
 
 Engine::~Engine() {
@@ -87,6 +214,10 @@ void Engine::free_device() {
   dfree(d_invb_);
   dfree(d_df_state_);
   dfree(d_acolptr_);
+  dfree(d_crow_);
+  dfree(d_ccol_);
+  dfree(d_csrc_);
+  dfree(d_cval_);
   dfree(d_arowind_);
   dfree(d_acolidx_);
   dfree(d_vals_);
@@ -100,6 +231,7 @@ void Engine::free_device() {
   dfree(d_x_);
   dfree(d_r_);
   dfree(d_red_);
+  dfree(d_part_);
   dfree(d_z_);
   dfree(d_p_);
   dfree(d_q_);
@@ -121,6 +253,9 @@ int64_t Engine::device_bytes(int nrhs) const {
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
   b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
+  if (opt_.precision == 1)  // full CSR of A (row pointers, columns, sources, values), at most 2 nnz
+    b += (int64_t)(n_ + 1) * 8 + 2 * nnzA_ * (4 + 8 + 8);
+  b += 2 * (int64_t)std::max(1, nrhs) * kRedBlocks * 8;
   b += (int64_t)n_ * std::max(1, nrhs) * (2 * elem + (opt_.precision == 0 ? 3 : 6) * 8) +
        48 * std::max(1, nrhs);
   return b;
@@ -203,6 +338,36 @@ void Engine::ensure_device() {
   }
   d_acolptr_ = dupload(h_colptr_, s);
   d_arowind_ = dupload(h_rowind_, s);
+  if (opt_.precision == 1) {
+    // full CSR of the symmetric matrix: entry (r,c) of the lower CSC goes to
+    // row r (column c) and, off the diagonal, to row c (column r)
+    std::vector<int64_t> rp(n_ + 1, 0);
+    for (Int c = 0; c < n_; ++c)
+      for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) {
+        const int32_t r = h_rowind_[p];
+        ++rp[r + 1];
+        if (r != c) ++rp[c + 1];
+      }
+    for (Int i = 0; i < n_; ++i) rp[i + 1] += rp[i];
+    nnz_full_ = rp[n_];
+    std::vector<int32_t> col(nnz_full_);
+    std::vector<int64_t> src(nnz_full_);
+    std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+    for (Int c = 0; c < n_; ++c)
+      for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) {
+        const int32_t r = h_rowind_[p];
+        col[fill[r]] = c;
+        src[fill[r]++] = p;
+        if (r != c) {
+          col[fill[c]] = r;
+          src[fill[c]++] = p;
+        }
+      }
+    d_crow_ = dupload(rp, s);
+    d_ccol_ = dupload(col, s);
+    d_csrc_ = dupload(src, s);
+    d_cval_ = dalloc<double>((size_t)nnz_full_);
+  }
   d_acolidx_ = dupload(colidx, s);
   d_vals_ = dalloc<double>(nnzA_);
   d_scale_ = dalloc<double>(n_);
@@ -221,6 +386,10 @@ void Engine::distribute(const double* nzval, bool on_device) {
   ensure_device();
   HIP_OK(hipMemcpyAsync(d_vals_, nzval, nnzA_ * sizeof(double),
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  if (d_cval_) {
+    k_gather_vals<<<grid_for(nnz_full_), 256, 0, stream_>>>(d_cval_, d_vals_, d_csrc_, nnz_full_);
+    HIP_OK(hipGetLastError());
+  }
   // ||A||_inf of the symmetric matrix and the equilibration scaling
   // s_i = 1/sqrt(|a_ii|) used by the FP32 factor.
   std::vector<double> vals;
@@ -606,6 +775,7 @@ void Engine::ensure_solve_capacity(int nrhs) {
   dfree(d_x_);
   dfree(d_r_);
   dfree(d_red_);
+  dfree(d_part_);
   dfree(d_z_);
   dfree(d_p_);
   dfree(d_q_);
@@ -618,6 +788,7 @@ void Engine::ensure_solve_capacity(int nrhs) {
   d_x_ = dalloc<double>(cnt);
   d_r_ = dalloc<double>(cnt);
   d_red_ = dalloc<double>(2 * (size_t)nrhs);
+  d_part_ = dalloc<double>(2 * (size_t)nrhs * kRedBlocks);
   if (opt_.precision == 1) {
     d_z_ = dalloc<double>(cnt);
     d_p_ = dalloc<double>(cnt);
@@ -737,84 +908,15 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   HIP_OK(hipGetLastError());
 }
 
-// r = b - A x (FP64, original numbering).
+// r = b - A x (FP64, original numbering; FP32 factor mode, which holds A as
+// full CSR), the residual check_solution forms (driver/utils.hpp:95-142).
 void Engine::residual(const double* b, const double* x, double* r, int nrhs) {
-  dev::k_copy<<<grid_for((int64_t)n_ * nrhs), 256, 0, stream_>>>(b, r, (int64_t)n_ * nrhs);
-  const int cols_per_block = 4;
-  unsigned g = (unsigned)((n_ + cols_per_block - 1) / cols_per_block);
-  dev::k_residual<<<g, 64 * cols_per_block, 0, stream_>>>(d_acolptr_, d_arowind_, d_vals_, x, r, n_, nrhs);
+  if (!d_cval_) throw std::logic_error("residual: the full CSR of A exists in FP32 factor mode only");
+  csr_mv(x, b, r, nrhs);
   HIP_OK(hipGetLastError());
 }
 
-// per-RHS max |r| and max |x| (column-major n x nrhs), one block per column
-__global__ void k_colmax2(const double* __restrict__ r, const double* __restrict__ x, int n,
-                          double* __restrict__ out) {
-  __shared__ double sr[256], sx[256];
-  int j = blockIdx.x;
-  double mr = 0.0, mx = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    mr = fmax(mr, fabs(r[(int64_t)j * n + i]));
-    mx = fmax(mx, fabs(x[(int64_t)j * n + i]));
-  }
-  sr[threadIdx.x] = mr;
-  sx[threadIdx.x] = mx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + s]);
-      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    out[2 * j] = sr[0];
-    out[2 * j + 1] = sx[0];
-  }
-}
 
-// y += A x for the symmetric matrix held as its lower CSC (column-major
-// n x nrhs vectors, original numbering).
-__global__ void k_symv(const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
-                       const double* __restrict__ val, const double* __restrict__ x,
-                       double* __restrict__ y, int n, int nrhs) {
-  int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (c >= n) return;
-  for (int j = 0; j < nrhs; ++j) {
-    const double* xj = x + (int64_t)j * n;
-    double* yj = y + (int64_t)j * n;
-    const double xc = xj[c];
-    double acc = 0.0;
-    for (int64_t p = colptr[c] + lane; p < colptr[c + 1]; p += 64) {
-      const int row = rowind[p];
-      const double v = val[p];
-      if (row != c) {
-        unsafeAtomicAdd(yj + row, v * xc);
-        acc += v * xj[row];
-      } else {
-        acc += v * xc;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if (lane == 0) unsafeAtomicAdd(yj + c, acc);
-  }
-}
-
-// out[j] = sum_i a[i,j] b[i,j], one block per column (fixed summation order).
-__global__ void k_coldot(const double* __restrict__ a, const double* __restrict__ b, int n,
-                         double* __restrict__ out) {
-  __shared__ double sh[256];
-  const int j = blockIdx.x;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += a[(int64_t)j * n + i] * b[(int64_t)j * n + i];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[j] = sh[0];
-}
 
 // CG step: alpha_j = rho_j / (p_j . q_j); x += alpha p; r -= alpha q.
 __global__ void k_pcg_xr(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
@@ -850,7 +952,7 @@ void Engine::ir_step(T* Y, T* X, int nrhs) {
   solve_sequence<T>(d_arena32_, Y, X, nrhs);
   dev::k_scatter_sol<T><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
   residual(d_b_, d_x_, d_r_, nrhs);
-  k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+  colmax2(d_r_, d_x_, nrhs, d_red_);
   HIP_OK(hipGetLastError());
 }
 
@@ -874,17 +976,14 @@ void Engine::pcg_iteration(int nrhs) {
   double* rho = d_s_;
   double* pq = d_s_ + nrhs;
   double* rho_new = d_s_ + 2 * nrhs;
-  dev::k_fill_zero<double><<<g, 256, 0, stream_>>>(d_q_, cnt);
-  const int cols_per_block = 4;
-  k_symv<<<(unsigned)((n_ + cols_per_block - 1) / cols_per_block), 64 * cols_per_block, 0, stream_>>>(
-      d_acolptr_, d_arowind_, d_vals_, d_p_, d_q_, n_, nrhs);
-  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_p_, d_q_, n_, pq);
+  csr_mv(d_p_, nullptr, d_q_, nrhs);
+  coldot(d_p_, d_q_, nrhs, pq);
   k_pcg_xr<<<g, 256, 0, stream_>>>(d_x_, d_r_, d_p_, d_q_, rho, pq, n_, nrhs);
   precond(d_r_, d_z_, nrhs);
-  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_z_, n_, rho_new);
+  coldot(d_r_, d_z_, nrhs, rho_new);
   k_pcg_p<<<g, 256, 0, stream_>>>(d_p_, d_z_, rho_new, rho, n_, nrhs);
   dev::k_copy<<<1, 256, 0, stream_>>>(rho_new, rho, (int64_t)nrhs);
-  k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+  colmax2(d_r_, d_x_, nrhs, d_red_);
   HIP_OK(hipGetLastError());
 }
 
@@ -893,7 +992,7 @@ void Engine::pcg_restart(int nrhs) {
   const int64_t cnt = (int64_t)n_ * nrhs;
   precond(d_r_, d_z_, nrhs);
   dev::k_copy<<<grid_for(cnt), 256, 0, stream_>>>(d_z_, d_p_, cnt);
-  k_coldot<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_z_, n_, d_s_);
+  coldot(d_r_, d_z_, nrhs, d_s_);
   HIP_OK(hipGetLastError());
 }
 
@@ -941,7 +1040,7 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
     };
     dev::k_fill_zero<double><<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_x_, (int64_t)cnt);
     dev::k_copy<<<grid_for((int64_t)cnt), 256, 0, stream_>>>(d_b_, d_r_, (int64_t)cnt);
-    k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_b_, d_x_, n_, d_red_);
+    colmax2(d_b_, d_x_, nrhs, d_red_);
     read_berr();
     for (int j = 0; j < nrhs; ++j) bnorm[j] = red[2 * j];
     rep.converged = 0;
@@ -955,7 +1054,7 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
         rep.iterations = it + 1;
         if (berr <= opt_.ir_tol || it + 1 == maxit) {
           residual(d_b_, d_x_, d_r_, nrhs);  // the true residual decides
-          k_colmax2<<<(unsigned)nrhs, 256, 0, stream_>>>(d_r_, d_x_, n_, d_red_);
+          colmax2(d_r_, d_x_, nrhs, d_red_);
           berr = read_berr();
           rep.backward_error = berr;
           if (berr <= opt_.ir_tol) {

@@ -13,6 +13,8 @@
 
 namespace spk {
 
+constexpr int kRedBlocks = 64;  // blocks per column of the refinement reductions
+
 struct EngineOptions {
   int ordering = ORD_MMD;
   int ldl = 0;            // 0: LL^T, 1: static LDL^T
@@ -111,6 +113,9 @@ class Engine {
   template <typename T>
   void assemble(T* arena);
   void residual(const double* b, const double* x, double* r, int nrhs);
+  void csr_mv(const double* x, const double* b, double* out, int nrhs);
+  void colmax2(const double* r, const double* x, int nrhs, double* out);
+  void coldot(const double* a, const double* b, int nrhs, double* out);
   void precond(const double* r, double* z, int nrhs);
   void pcg_iteration(int nrhs);
   void pcg_restart(int nrhs);
@@ -195,6 +200,15 @@ class Engine {
   double* d_x_ = nullptr;
   double* d_r_ = nullptr;
   double* d_red_ = nullptr;
+  // A as full CSR (both triangles, original numbering) for the refinement
+  // residuals and CG products (FP32 factor mode): row pointers, columns,
+  // source entry in the caller's lower CSC, values gathered from it
+  int64_t* d_crow_ = nullptr;
+  int32_t* d_ccol_ = nullptr;
+  int64_t* d_csrc_ = nullptr;
+  double* d_cval_ = nullptr;
+  int64_t nnz_full_ = 0;
+  double* d_part_ = nullptr;  // per-block partial reductions, nrhs x kRedBlocks x 2
   double* d_z_ = nullptr;  // preconditioned CG vectors (FP32 factor mode)
   double* d_p_ = nullptr;
   double* d_q_ = nullptr;

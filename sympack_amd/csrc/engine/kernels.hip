@@ -1631,36 +1631,5 @@ __global__ void k_scatter_sol(const T* __restrict__ X, double* __restrict__ x,
   }
 }
 
-// ---------------------------------------------------------------------------
-// FP64 residual r = b - A x with A given by its lower triangle (CSC, 0-based)
-// in the original numbering, as check_solution does (driver/utils.hpp:95-142):
-// one workgroup-stride loop per column, symmetric contributions by atomics.
-// ---------------------------------------------------------------------------
-__global__ void k_residual(const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
-                           const double* __restrict__ val, const double* __restrict__ x,
-                           double* __restrict__ r, int n, int nrhs) {
-  int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (c >= n) return;
-  for (int j = 0; j < nrhs; ++j) {
-    const double* xj = x + (int64_t)j * n;
-    double* rj = r + (int64_t)j * n;
-    double xc = xj[c];
-    double acc = 0.0;
-    for (int64_t p = colptr[c] + lane; p < colptr[c + 1]; p += 64) {
-      int row = rowind[p];
-      double v = val[p];
-      if (row != c) {
-        unsafeAtomicAdd(rj + row, -v * xc);  // A(row,c) x(c)
-        acc += v * xj[row];             // A(c,row) x(row)
-      } else {
-        acc += v * xc;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if (lane == 0) unsafeAtomicAdd(rj + c, -acc);
-  }
-}
-
 }  // namespace dev
 }  // namespace spk
