@@ -594,7 +594,7 @@ static void diag_launch(unsigned g, hipStream_t st, const SnInfo* sn, const Task
 template <typename T, bool LDL, int NBM>
 static void ptrsm_launch(unsigned g, hipStream_t st, const SnInfo* sn, const Task4* t, T* arena,
                          const T* dinv) {
-  dev::k_ptrsm<T, LDL, NBM><<<g, 256, 0, st>>>(sn, t, arena, dinv);
+  dev::k_ptrsm<T, LDL, NBM><<<g, 512, 0, st>>>(sn, t, arena, dinv);
 }
 
 template <typename T>

@@ -528,13 +528,15 @@ __global__ __launch_bounds__(NT) void k_diag(const SnInfo* __restrict__ sn,
 // of tile J is k-chunk kk), so no data moves between the steps.
 // Task {sn, step, row_tile, inverse-tile offset}.
 // ---------------------------------------------------------------------------
-template <typename T, bool LDL, int NBM>
+template <typename T, bool LDL, int NBM, int ROWS>
 __device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, int nb, int rt,
                                               T* __restrict__ P, const T* __restrict__ tl,
                                               const T* __restrict__ dd);
 
+// 128 rows per workgroup (8 waves): the LDS-resident block (up to 119 KB)
+// allows one workgroup per CU, so each holds twice the rows of a 4-wave one
 template <typename T, bool LDL, int NBM>
-__global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
+__global__ __launch_bounds__(2 * NT) void k_ptrsm(const SnInfo* __restrict__ sn,
                                                const Task4* __restrict__ tasks,
                                                T* __restrict__ arena, const T* __restrict__ dinv) {
   typedef typename Mfma<T>::v4 v4;
@@ -548,8 +550,8 @@ __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
   const int nb = (bs + SUB - 1) / SUB;
   const int64_t ld = s.r;
   T* __restrict__ P = arena + s.off + (int64_t)c0 * ld;  // column c0 of the panel
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  {
+  const int tid = threadIdx.x;
+  if (tid < NT) {  // the first 256 threads stage the block
     const int r = tid & 15, c = tid >> 4;
     const T* Dv = dinv + (int64_t)t.d * (SUB * SUB);
     T v[NTL];
@@ -581,13 +583,13 @@ __global__ __launch_bounds__(NT) void k_ptrsm(const SnInfo* __restrict__ sn,
     }
   }
   __syncthreads();
-  ptrsm_compute<T, LDL, NBM>(s, c0, bs, nb, t.c, P, tl, dd);
+  ptrsm_compute<T, LDL, NBM, kTrsmRowsWide>(s, c0, bs, nb, t.c, P, tl, dd);
 }
 
 // The TRSM proper: rows of row tile rt below the block, with the strict
 // lower tiles and the inverse diagonal tiles in LDS (tl) and, for LDL, 1/D
 // of every column (dd).  P points at column c0 of the panel.
-template <typename T, bool LDL, int NBM>
+template <typename T, bool LDL, int NBM, int ROWS>
 __device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, int nb, int rt,
                                               T* __restrict__ P, const T* __restrict__ tl,
                                               const T* __restrict__ dd) {
@@ -595,7 +597,7 @@ __device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, i
   const int64_t ld = s.r;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // rows of this wave: R0 + (lane & 15), columns as MFMA rows
-  const int R0 = c0 + bs + rt * kTrsmRows + 16 * wave;
+  const int R0 = c0 + bs + rt * ROWS + 16 * wave;
   if (R0 >= s.r) return;  // no barrier follows
   const int prow = min(R0 + (lane & 15), s.r - 1);
   const bool prow_ok = R0 + (lane & 15) < s.r;
@@ -670,7 +672,7 @@ __global__ __launch_bounds__(NT) void k_diagtrsm(const SnInfo* __restrict__ sn,
   const SnInfo s = sn[tk.a];
   diag_body<T, LDL, NBM, true>(t, s, arena, dinv, info, L, cnt + tk.a, tk.b);
   const int bs = min(kStep, s.w);
-  ptrsm_compute<T, LDL, NBM>(s, 0, bs, (bs + SUB - 1) / SUB, tk.c, arena + s.off, L.tl, L.ysh);
+  ptrsm_compute<T, LDL, NBM, kTrsmRows>(s, 0, bs, (bs + SUB - 1) / SUB, tk.c, arena + s.off, L.tl, L.ysh);
 }
 
 // ---------------------------------------------------------------------------

@@ -265,7 +265,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
         int64_t rows_below = si.r - c0 - bs;
         int32_t inv_off = (int32_t)(p.inv_base[I] + (int64_t)st * (kStep / kSub));
         p.potrf_tasks.push_back({I, st, inv_off, 0});
-        for (int64_t rt = 0; rt < ceil_div(rows_below, kTrsmRows); ++rt)
+        for (int64_t rt = 0; rt < ceil_div(rows_below, kTrsmRowsWide); ++rt)
           p.trsm_tasks.push_back({I, st, (int32_t)rt, inv_off});
         p.diag_flops += (double)bs * bs * bs / 3.0 + (double)bs * bs * rows_below;
         if (st < nsteps - 1) {

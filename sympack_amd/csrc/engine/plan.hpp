@@ -24,7 +24,8 @@ constexpr int kNB = 64;      // diagonal block width of the blocked solve
 constexpr int kTile = 64;    // update tile edge (rows x cols)
 constexpr int kStep = 160;   // panel columns factored per step (diagonal block <= 160 x 160)
 constexpr int kSub = 16;     // sub-block edge inside a diagonal block (MFMA 16x16 tiles)
-constexpr int kTrsmRows = 64;  // panel rows per TRSM workgroup (16 per wave)
+constexpr int kTrsmRows = 64;  // panel rows per fused block factor + TRSM workgroup (16 per wave)
+constexpr int kTrsmRowsWide = 128;  // panel rows per k_ptrsm workgroup (8 waves of 16 rows)
 constexpr int kSchurKC = 32;   // k columns per LDS chunk of the Schur update
 // Schur launches with fewer tasks than this (one per CU of the 256) split
 // each task's k chunks over several workgroups; more splitting costs more in
