@@ -69,6 +69,10 @@ typedef struct spk_options {
   int solve_dataflow; /* must be 0 (default): one launch per etree level and kernel kind.
                          The dependency-driven sweeps (one persistent launch each) are
                          disabled in this build; Solve rejects 1 */
+  int refine;         /* supernode reordering (symPACKOptions.order_refinement_str,
+                         symPACKMatrix2D.hpp:3090-3111): 0 = NO (default), or
+                         10*ordflag + altflag: 10, 11, 20, 21, 30, 31 for SET10 .. SET31
+                         (SET = 31) */
 } spk_options;
 
 typedef struct spk_stats {

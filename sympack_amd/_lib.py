@@ -35,6 +35,7 @@ class SpkOptions(ctypes.Structure):
         ("device", ctypes.c_int),
         ("merge_updates", ctypes.c_int),
         ("solve_dataflow", ctypes.c_int),
+        ("refine", ctypes.c_int),
     ]
 
 

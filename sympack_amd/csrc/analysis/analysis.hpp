@@ -122,9 +122,17 @@ struct Symbolic {
 // (include/sympack/symPACKMatrix2D.hpp:2699-3081).  The input is the
 // caller's lower-triangular CSC pattern (incl. diagonal).
 // user_perm: for ORD_USER only (any base; shifted by its minimum).
+// refine: 0, or 10*ordflag + altflag of the supernode reordering
+// (-refine SET10 .. SET31; symPACKMatrix2D.hpp:3090-3111).
 void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int baseval,
                             int ordering, const RelaxParams& relax, const Int* user_perm,
-                            Symbolic& out);
+                            Symbolic& out, int refine = 0);
+
+// Reordering inside the supernodes (refine.cpp): new position of every
+// column (1-based), to be composed into the ordering.
+std::vector<Int> refine_supernodes(int ordflag, int altflag, Int n, const std::vector<Int>& xsuper,
+                                   const std::vector<Int>& snode, const std::vector<Ptr>& xlindx,
+                                   const std::vector<Idx>& lindx);
 
 // Pieces, exposed for tests.
 void lcol_row_count(const Graph& sg, const ETree& et, const Ordering& ord,

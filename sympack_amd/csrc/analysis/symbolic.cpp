@@ -362,7 +362,8 @@ void symbolic_structure(const Graph& expanded, const Ordering& ord, const ETree&
 }
 
 void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int baseval, int ordering,
-                            const RelaxParams& relax, const Int* user_perm, Symbolic& out) {
+                            const RelaxParams& relax, const Int* user_perm, Symbolic& out,
+                            int refine) {
   if (n <= 0) throw std::invalid_argument("matrix dimension must be positive");
   out = Symbolic();
   out.n = n;
@@ -450,6 +451,17 @@ void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int bas
   bool first_col_only = (relax.nrelax0 == 0);
   symbolic_structure(graph, ord, et, cc, out.xsuper, out.supMembership, first_col_only, out.xlindx,
                      out.lindx);
+  if (refine > 0) {
+    // symPACKMatrix2D.hpp:3090-3111: reorder inside the supernodes, compose
+    // (Ordering::Compose, src/Ordering.cpp:1301-1314), and build the row
+    // structure again with the same partition (3246-3254)
+    const std::vector<Int> invp2 = refine_supernodes(refine / 10, refine % 10, n, out.xsuper,
+                                                     out.supMembership, out.xlindx, out.lindx);
+    for (Int i = 0; i < n; ++i) ord.invp[i] = invp2[ord.invp[i] - 1];
+    for (Int i = 0; i < n; ++i) ord.perm[ord.invp[i] - 1] = i + 1;
+    symbolic_structure(graph, ord, et, cc, out.xsuper, out.supMembership, first_col_only,
+                       out.xlindx, out.lindx);
+  }
 
   // supernodal etree (ETree::ToSupernodalETree, src/ETree.cpp:400-426)
   Int ns = out.nsuper();

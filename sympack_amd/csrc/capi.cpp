@@ -74,6 +74,7 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   e.device = o.device;
   e.merge_updates = o.merge_updates;
   e.solve_dataflow = o.solve_dataflow;
+  e.refine = o.refine;
   return e;
 }
 }  // namespace
@@ -130,6 +131,7 @@ void spk_default_options(spk_options* o) {
   o->device = g_device;
   o->merge_updates = 1;
   o->solve_dataflow = 0;
+  o->refine = 0;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -187,6 +189,19 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         o->ir_max_iter = std::atoi(val(1).c_str());
       } else if (k == "-merge") {
         o->merge_updates = std::atoi(val(1).c_str());
+      } else if (k == "-refine") {
+        // symPACKMatrix2D.hpp:3090-3111 (utils.hpp:331-333): SET = SET31;
+        // other names leave the ordering as it is
+        std::string v = val(1);
+        if (v == "SET" || v == "SET31") o->refine = 31;
+        else if (v == "SET10") o->refine = 10;
+        else if (v == "SET11") o->refine = 11;
+        else if (v == "SET20") o->refine = 20;
+        else if (v == "SET21") o->refine = 21;
+        else if (v == "SET30") o->refine = 30;
+        else if (v.compare(0, 4, "TSPB") == 0)
+          throw std::logic_error("-refine TSPB is not supported; use SET, SET10 .. SET31");
+        else o->refine = 0;
       } else if (k == "-dataflow") {
         o->solve_dataflow = std::atoi(val(1).c_str()) != 0;
       }

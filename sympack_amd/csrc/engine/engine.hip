@@ -276,7 +276,8 @@ void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) 
   free_device();
   have_symbolic_ = have_values_ = factored_ = device_ready_ = false;
   const Int* up = opt_.user_perm.empty() ? nullptr : opt_.user_perm.data();
-  symbolic_factorization(n, colptr, rowind, baseval, opt_.ordering, opt_.relax, up, sym_);
+  symbolic_factorization(n, colptr, rowind, baseval, opt_.ordering, opt_.relax, up, sym_,
+                         opt_.refine);
   build_plan(sym_, n, colptr, rowind, baseval, plan_, opt_.merge_updates);
   n_ = n;
   nnzA_ = colptr[n] - baseval;

@@ -27,6 +27,7 @@ struct EngineOptions {
   int verbose = 0;
   int merge_updates = 1;  // 0 off, 1 on levels that fill the GPU, 2 every qualifying chain child
   int solve_dataflow = 0; // dependency-driven solve sweeps: disabled, requesting it is an error
+  int refine = 0;         // 10*ordflag + altflag of the supernode reordering, 0 = none
   std::vector<Int> user_perm;
 };
 

@@ -50,7 +50,8 @@ class symPACKOptions:
 
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
-                 profile=False, device=0, merge_updates=1, solve_dataflow=False):
+                 profile=False, device=0, merge_updates=1, solve_dataflow=False,
+                 order_refinement_str="NO"):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -63,6 +64,9 @@ class symPACKOptions:
         self.device = device
         self.merge_updates = merge_updates
         self.solve_dataflow = solve_dataflow
+        # supernode reordering (symPACKMatrix2D.hpp:3090-3111): NO, SET,
+        # SET10, SET11, SET20, SET21, SET30, SET31
+        self.order_refinement_str = order_refinement_str
 
     @classmethod
     def from_args(cls, args):
@@ -78,7 +82,8 @@ class symPACKOptions:
         return cls(ordering=inv[o.ordering], decomposition="LDL" if o.decomposition else "LL",
                    relax=r, precision="FP32_IR" if o.precision else "FP64",
                    ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
-                   merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow))
+                   merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow),
+                   order_refinement_str="SET%d" % o.refine if o.refine else "NO")
 
     def to_c(self):
         lib = _lib.load()
@@ -101,6 +106,11 @@ class symPACKOptions:
         o.device = self.device
         o.merge_updates = int(self.merge_updates)
         o.solve_dataflow = 1 if self.solve_dataflow else 0
+        r = self.order_refinement_str
+        if r.startswith("TSPB"):
+            raise ValueError("-refine TSPB is not supported; use SET, SET10 .. SET31")
+        o.refine = {"SET": 31, "SET10": 10, "SET11": 11, "SET20": 20, "SET21": 21,
+                    "SET30": 30, "SET31": 31}.get(r, 0)
         return o
 
 
