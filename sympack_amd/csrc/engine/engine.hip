@@ -47,6 +47,8 @@ inline unsigned grid_for(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
 }
+// k_gather_rhs / k_scatter_sol: one workgroup per 64 rows
+inline unsigned grid_rows64(int n) { return (unsigned)std::max(1, (n + 63) / 64); }
 enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED };
 }  // namespace
 
@@ -134,26 +136,41 @@ __global__ void k_gather_vals(double* __restrict__ cval, const double* __restric
 }
 
 // out = b - A x (b given) or out = A x, A symmetric as full CSR, vectors
-// column-major n x nrhs.  One wave per row: lane (e, j) = (lane / 16,
-// lane % 16) sums entries e, e + 4, ... of the row for right-hand side j of
-// each group of 16; no atomics, a fixed summation order.
-__global__ void k_csr_mv(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                         const double* __restrict__ val, const double* __restrict__ x,
-                         const double* __restrict__ b, double* __restrict__ out, int n, int nrhs) {
-  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+// column-major n x nrhs.  One thread per row, 16 right-hand sides per pass
+// over the row's entries (each entry read once per 16 right-hand sides, 16
+// independent x loads in flight): the lanes of a wave hold 64 consecutive
+// rows, so for a banded (grid) matrix an entry position reads 64 nearly
+// consecutive x values of each column.  The entries of a row are summed in
+// CSR order; no atomics.
+__global__ __launch_bounds__(256) void k_csr_mv(const int64_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col,
+                                                const double* __restrict__ val,
+                                                const double* __restrict__ x,
+                                                const double* __restrict__ b,
+                                                double* __restrict__ out, int n, int nrhs) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
-  const int lane = threadIdx.x & 63, e = lane >> 4, j16 = lane & 15;
   const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
-  for (int j0 = 0; j0 < nrhs; j0 += 16) {
-    const int j = j0 + j16;
-    const double* __restrict__ xj = x + (int64_t)std::min(j, nrhs - 1) * n;
-    double acc = 0.0;
-    for (int64_t p = p0 + e; p < p1; p += 4) acc += val[p] * xj[col[p]];
-    acc += __shfl_xor(acc, 16);
-    acc += __shfl_xor(acc, 32);
-    if (e == 0 && j < nrhs) {
-      const int64_t o = (int64_t)j * n + row;
-      out[o] = b ? b[o] - acc : acc;
+  constexpr int JC = 16;
+  for (int j0 = 0; j0 < nrhs; j0 += JC) {
+    const int jn = min(JC, nrhs - j0);
+    const double* __restrict__ xj = x + (int64_t)j0 * n;
+    double acc[JC];
+#pragma unroll
+    for (int q = 0; q < JC; ++q) acc[q] = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+      const double v = val[p];
+      const double* __restrict__ xc = xj + col[p];
+#pragma unroll
+      for (int q = 0; q < JC; ++q)
+        if (q < jn) acc[q] += v * xc[(int64_t)q * n];
+    }
+#pragma unroll
+    for (int q = 0; q < JC; ++q) {
+      if (q < jn) {
+        const int64_t o = (int64_t)(j0 + q) * n + row;
+        out[o] = b ? b[o] - acc[q] : acc[q];
+      }
     }
   }
 }
@@ -173,8 +190,7 @@ void Engine::coldot(const double* a, const double* b, int nrhs, double* out) {
 
 // y = A x on the full CSR (FP64, original numbering)
 void Engine::csr_mv(const double* x, const double* b, double* out, int nrhs) {
-  const unsigned g = (unsigned)((n_ + 3) / 4);
-  k_csr_mv<<<g, 256, 0, stream_>>>(d_crow_, d_ccol_, d_cval_, x, b, out, n_, nrhs);
+  k_csr_mv<<<grid_for(n_), 256, 0, stream_>>>(d_crow_, d_ccol_, d_cval_, x, b, out, n_, nrhs);
 }
 
 Engine::Engine() { kind_stats.resize(10); }
@@ -200,7 +216,7 @@ void Engine::free_device() {
   dfree(d_merges_);
   dfree(d_mmap_);
   dfree(d_amap_);
-  dfree(d_perm0_);
+  dfree(d_invp0_);
   for (auto& t : d_tasks_) dfree(t);
   dfree(d_info_);
   dfree(d_fcnt_);
@@ -308,7 +324,13 @@ void Engine::ensure_device() {
   d_merges_ = dupload(plan_.merges, s);
   d_mmap_ = dupload(plan_.mmap, s);
   d_amap_ = dupload(plan_.amap, s);
-  d_perm0_ = dupload(plan_.perm0, s);
+  {
+    // invp0[old] = new, 0-based: the right-hand side gather and the solution
+    // scatter walk the original numbering
+    std::vector<int32_t> invp0(plan_.perm0.size());
+    for (size_t i = 0; i < invp0.size(); ++i) invp0[plan_.perm0[i]] = (int32_t)i;
+    d_invp0_ = dupload(invp0, s);
+  }
   d_tasks_[TA_POTRF] = dupload(plan_.potrf_tasks, s);
   d_tasks_[TA_TRSM] = dupload(plan_.trsm_tasks, s);
   d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
@@ -948,10 +970,9 @@ __global__ void k_pcg_p(double* __restrict__ p, const double* __restrict__ z,
 // x += d, r = b - A x and the per-RHS norms, all on the device.
 template <typename T>
 void Engine::ir_step(T* Y, T* X, int nrhs) {
-  const unsigned g = grid_for((int64_t)n_ * nrhs);
-  dev::k_gather_rhs<T><<<g, 256, 0, stream_>>>(d_r_, Y, d_perm0_, n_, nrhs, d_scale_);
+  dev::k_gather_rhs<T><<<grid_rows64(n_), 256, 0, stream_>>>(d_r_, Y, d_invp0_, n_, nrhs, d_scale_);
   solve_sequence<T>(d_arena32_, Y, X, nrhs);
-  dev::k_scatter_sol<T><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, d_scale_, 1);
+  dev::k_scatter_sol<T><<<grid_rows64(n_), 256, 0, stream_>>>(X, d_x_, d_invp0_, n_, nrhs, d_scale_, 1);
   residual(d_b_, d_x_, d_r_, nrhs);
   colmax2(d_r_, d_x_, nrhs, d_red_);
   HIP_OK(hipGetLastError());
@@ -959,12 +980,11 @@ void Engine::ir_step(T* Y, T* X, int nrhs) {
 
 // z = S (L L^T)^{-1} S r with the FP32 factor of S A S (FP64 in and out).
 void Engine::precond(const double* r, double* z, int nrhs) {
-  const unsigned g = grid_for((int64_t)n_ * nrhs);
   float* Y = (float*)d_Y_;
   float* X = (float*)d_X_;
-  dev::k_gather_rhs<float><<<g, 256, 0, stream_>>>(r, Y, d_perm0_, n_, nrhs, d_scale_);
+  dev::k_gather_rhs<float><<<grid_rows64(n_), 256, 0, stream_>>>(r, Y, d_invp0_, n_, nrhs, d_scale_);
   solve_sequence<float>(d_arena32_, Y, X, nrhs);
-  dev::k_scatter_sol<float><<<g, 256, 0, stream_>>>(X, z, d_perm0_, n_, nrhs, d_scale_, 0);
+  dev::k_scatter_sol<float><<<grid_rows64(n_), 256, 0, stream_>>>(X, z, d_invp0_, n_, nrhs, d_scale_, 0);
   HIP_OK(hipGetLastError());
 }
 
@@ -1011,14 +1031,13 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   HIP_OK(hipMemcpyAsync(d_b_, rhs, cnt * sizeof(double),
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
   SolveReport rep;
-  const unsigned g = grid_for((int64_t)cnt);
   if (opt_.precision == 0) {
     double* Y = (double*)d_Y_;
     double* X = (double*)d_X_;
     run_graph(solve_graph(4 * (int64_t)nrhs), 4 * (int64_t)nrhs, [&] {
-      dev::k_gather_rhs<double><<<g, 256, 0, stream_>>>(d_b_, Y, d_perm0_, n_, nrhs, nullptr);
+      dev::k_gather_rhs<double><<<grid_rows64(n_), 256, 0, stream_>>>(d_b_, Y, d_invp0_, n_, nrhs, nullptr);
       solve_sequence<double>(d_arena64_, Y, X, nrhs);
-      dev::k_scatter_sol<double><<<g, 256, 0, stream_>>>(X, d_x_, d_perm0_, n_, nrhs, nullptr, 0);
+      dev::k_scatter_sol<double><<<grid_rows64(n_), 256, 0, stream_>>>(X, d_x_, d_invp0_, n_, nrhs, nullptr, 0);
       HIP_OK(hipGetLastError());
     });
   } else {

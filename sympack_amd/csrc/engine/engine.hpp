@@ -170,7 +170,7 @@ class Engine {
   MergeInfo* d_merges_ = nullptr;
   int32_t* d_mmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
-  int32_t* d_perm0_ = nullptr;
+  int32_t* d_invp0_ = nullptr;  // original -> permuted row, 0-based
   void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused
   int* d_info_ = nullptr;
   int* d_fcnt_ = nullptr;  // per supernode: loads of the fused block factor + TRSM

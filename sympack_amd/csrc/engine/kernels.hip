@@ -1599,37 +1599,72 @@ __global__ __launch_bounds__(NT) void k_sdiag_bwd(const SnInfo* __restrict__ sn,
   }
 }
 
-// b (column-major, original numbering) -> Y (row-major, permuted numbering)
+// b (column-major, original numbering) -> Y (row-major, permuted numbering).
+// One workgroup (4 waves) per 64 consecutive original rows, through an LDS
+// tile of 64 rows x 64 right-hand sides: a wave reads one column of b over
+// the 64 rows (contiguous), the workgroup then writes each row's right-hand
+// sides to its permuted row of Y (contiguous).  invp0[old] = new.
+// Grid: ceil(n / 64).
 template <typename T>
-__global__ void k_gather_rhs(const double* __restrict__ b, T* __restrict__ Y,
-                             const int32_t* __restrict__ perm0, int n, int nrhs,
-                             const double* __restrict__ esc_row) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
-    int o = perm0[i];
-    double v = b[(int64_t)j * n + o];
-    if (esc_row) v *= esc_row[o];
-    Y[idx] = (T)v;
+__global__ __launch_bounds__(256) void k_gather_rhs(const double* __restrict__ b,
+                                                    T* __restrict__ Y,
+                                                    const int32_t* __restrict__ invp0, int n,
+                                                    int nrhs, const double* __restrict__ esc_row) {
+  __shared__ T tile[64][65];
+  const int o0 = blockIdx.x * 64, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int o = o0 + lane;
+  const int rows = min(64, n - o0);
+  const double s = (esc_row && o < n) ? esc_row[o] : 1.0;
+  for (int j0 = 0; j0 < nrhs; j0 += 64) {
+    const int jn = min(64, nrhs - j0);
+    if (o < n) {
+      for (int jj = wave; jj < jn; jj += 4) {
+        double v = b[(int64_t)(j0 + jj) * n + o];
+        if (esc_row) v *= s;
+        tile[lane][jj] = (T)v;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < rows * jn; e += 256) {
+      const int r = e / jn, c = e - r * jn;
+      Y[(int64_t)invp0[o0 + r] * nrhs + j0 + c] = tile[r][c];
+    }
+    __syncthreads();
   }
 }
 
-// X (row-major, permuted) -> x (column-major, original numbering)
+// X (row-major, permuted) -> x (column-major, original numbering), the
+// transpose of k_gather_rhs's access pattern.  Grid: ceil(n / 64).
 template <typename T>
-__global__ void k_scatter_sol(const T* __restrict__ X, double* __restrict__ x,
-                              const int32_t* __restrict__ perm0, int n, int nrhs,
-                              const double* __restrict__ esc_row, int accumulate) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
-    int o = perm0[i];
-    double v = (double)X[idx];
-    if (esc_row) v *= esc_row[o];
-    double* dst = x + (int64_t)j * n + o;
-    if (accumulate)
-      *dst += v;
-    else
-      *dst = v;
+__global__ __launch_bounds__(256) void k_scatter_sol(const T* __restrict__ X,
+                                                     double* __restrict__ x,
+                                                     const int32_t* __restrict__ invp0, int n,
+                                                     int nrhs, const double* __restrict__ esc_row,
+                                                     int accumulate) {
+  __shared__ T tile[64][65];
+  const int o0 = blockIdx.x * 64, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int o = o0 + lane;
+  const int rows = min(64, n - o0);
+  const double s = (esc_row && o < n) ? esc_row[o] : 1.0;
+  for (int j0 = 0; j0 < nrhs; j0 += 64) {
+    const int jn = min(64, nrhs - j0);
+    for (int e = tid; e < rows * jn; e += 256) {
+      const int r = e / jn, c = e - r * jn;
+      tile[r][c] = X[(int64_t)invp0[o0 + r] * nrhs + j0 + c];
+    }
+    __syncthreads();
+    if (o < n) {
+      for (int jj = wave; jj < jn; jj += 4) {
+        double v = (double)tile[lane][jj];
+        if (esc_row) v *= s;
+        double* dst = x + (int64_t)(j0 + jj) * n + o;
+        if (accumulate)
+          *dst += v;
+        else
+          *dst = v;
+      }
+    }
+    __syncthreads();
   }
 }
 
