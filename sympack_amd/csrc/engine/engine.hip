@@ -158,12 +158,20 @@ __global__ __launch_bounds__(256) void k_csr_mv(const int64_t* __restrict__ rowp
     double acc[JC];
 #pragma unroll
     for (int q = 0; q < JC; ++q) acc[q] = 0.0;
+    // the next entry's value and column are loaded before this entry's x
+    // values are used, so the column-index round trip overlaps them
+    double v = p0 < p1 ? val[p0] : 0.0;
+    int32_t c = p0 < p1 ? col[p0] : 0;
     for (int64_t p = p0; p < p1; ++p) {
-      const double v = val[p];
-      const double* __restrict__ xc = xj + col[p];
+      const bool more = p + 1 < p1;
+      const double vn = more ? val[p + 1] : 0.0;
+      const int32_t cn = more ? col[p + 1] : 0;
+      const double* __restrict__ xc = xj + c;
 #pragma unroll
       for (int q = 0; q < JC; ++q)
         if (q < jn) acc[q] += v * xc[(int64_t)q * n];
+      v = vn;
+      c = cn;
     }
 #pragma unroll
     for (int q = 0; q < JC; ++q) {
