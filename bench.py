@@ -91,6 +91,7 @@ def precision_leg(k, nrhs, reps):
         out[key + "_backward_error"] = onum.backward_error(A, X[prec], B)
         if prec == "FP32_IR":
             out["ir_iterations"] = st["ir_iterations"]
+            out["ir_converged"] = bool(st["ir_converged"])
         out["flops_sum_cc2"] = st["flops"]
         del M
     out["speedup_fp32_ir_vs_fp64"] = round(out["fp64_ms"] / out["fp32_ir_ms"], 3)
@@ -154,6 +155,7 @@ def ldl_leg(k, nrhs, reps):
         out[key + "_backward_error"] = onum.backward_error(A, X[prec], B)
         if prec == "FP32_IR":
             out["ir_iterations"] = st["ir_iterations"]
+            out["ir_converged"] = bool(st["ir_converged"])
         out["flops_sum_cc2"] = st["flops"]
         del M
     out["forward_diff_fp32_ir_vs_fp64"] = float(np.linalg.norm(X["FP32_IR"] - X["FP64"]) /

@@ -89,6 +89,8 @@ typedef struct spk_stats {
   double backward_error;          /* last Solve (FP32+IR), -1 if not computed */
   double fp32_shift;              /* diagonal shift of the equilibrated matrix in the last
                                      FP32 factor (0 unless its Cholesky broke down) */
+  int ir_converged;               /* last Solve: 1 if the refinement reached ir_tol (always 1
+                                     for FP64), 0 if it stopped after ir_max_iter steps */
 } spk_stats;
 
 /* Per-kernel-kind event timings (profile mode).  kind: 0 diagonal factor,

@@ -59,6 +59,7 @@ class SpkStats(ctypes.Structure):
         ("ir_iterations", ctypes.c_int),
         ("backward_error", ctypes.c_double),
         ("fp32_shift", ctypes.c_double),
+        ("ir_converged", ctypes.c_int),
     ]
 
 

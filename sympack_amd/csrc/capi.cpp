@@ -336,6 +336,7 @@ int spk_get_stats(spk_handle h, spk_stats* st) {
     st->ir_iterations = h->engine.last_report.iterations;
     st->backward_error = h->engine.last_report.backward_error;
     st->fp32_shift = h->engine.fp32_shift_;
+    st->ir_converged = h->engine.last_report.converged;
     return 0;
   });
 }

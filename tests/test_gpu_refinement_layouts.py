@@ -39,3 +39,23 @@ def test_fp32_ir_partial_tiles_many_rhs(name, gen, ldl):
     assert onum.backward_error(A, X, B) <= 1e-12
     Xo = onum.factor_and_solve(Mx, A, B, ldl)
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-8
+
+
+def test_refinement_reports_convergence():
+    n, cp, ri, v = M.shifted_laplacian_3d_7pt(10, 0.3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    B = np.random.default_rng(4).uniform(-1, 1, (n, 2))
+    for maxit, tol, want in [(30, 1e-12, 1), (1, 1e-30, 0)]:
+        Mx = sp.symPACKMatrix2D()
+        Mx.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", precision="FP32_IR",
+                                  ir_tol=tol, ir_max_iter=maxit))
+        Mx.SymbolicFactorization(A)
+        Mx.DistributeMatrix(A)
+        Mx.Factorize()
+        Mx.Solve(B, 2)
+        st = Mx.stats()
+        assert st["ir_converged"] == want
+        if want:
+            assert st["backward_error"] <= tol
+        else:
+            assert st["ir_iterations"] == maxit and st["backward_error"] > tol
