@@ -17,9 +17,17 @@ Follows, function by function:
   boxnd / gridnd     src/boxnd.f, src/gridnd.f
 Pure-Python loops: small problems only (n up to a few thousand).
 
-Parity pins: etree and column counts are checked against brute-force
-symbolic elimination; orderings against the compiled reference Fortran
-(oracle/_ref) golden vectors; the whole chain against the C++ product.
+Parity pins: orderings against the compiled reference Fortran
+(oracle/_ref) golden vectors; etree and column counts against brute-force
+symbolic elimination (tests/test_symbolic.py); postorder, SortChildren
+(the whole final ordering rebuilt with explicit child lists), fundamental
+supernodes, the relaxed partition's structure (unions of fundamental
+supernodes along parent chains, width caps, every merge of size <= nrelax0
+taken) and the relaxed row structures against a dense boolean elimination
+(tests/test_symbolic_invariants.py).  Parity unpinned: the z-threshold
+decisions of relax_supernodes for merges wider than nrelax0
+(impl:529-566) follow the reference's arithmetic and are checked only
+against the C++ restatement.
 """
 import math
 

@@ -74,6 +74,13 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   e.device = o.device;
   e.merge_updates = o.merge_updates;
   e.solve_dataflow = o.solve_dataflow;
+  // the reordering codes ordsup takes: 10 * ordflag + altflag
+  // (symPACKMatrix2D.hpp:3090-3111); anything else is an argument error
+  if (o.refine != 0 && o.refine != 10 && o.refine != 11 && o.refine != 20 && o.refine != 21 &&
+      o.refine != 30 && o.refine != 31)
+    throw std::logic_error("refine must be 0, 10, 11, 20, 21, 30 or 31, got " + std::to_string(o.refine));
+  if (o.precision != SPK_PREC_FP64 && o.precision != SPK_PREC_FP32_IR)
+    throw std::logic_error("precision must be SPK_PREC_FP64 or SPK_PREC_FP32_IR");
   e.refine = o.refine;
   return e;
 }
@@ -182,7 +189,12 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         o->verbose = std::atoi(val(1).c_str());
       } else if (k == "-precision") {
         std::string v = val(1);
-        o->precision = (v == "FP32" || v == "fp32" || v == "FP32IR") ? SPK_PREC_FP32_IR : SPK_PREC_FP64;
+        if (v == "FP64" || v == "fp64")
+          o->precision = SPK_PREC_FP64;
+        else if (v == "FP32" || v == "fp32" || v == "FP32IR" || v == "FP32_IR" || v == "fp32_ir")
+          o->precision = SPK_PREC_FP32_IR;
+        else
+          throw std::logic_error("-precision must be FP64 or FP32_IR (FP32, FP32IR), got '" + v + "'");
       } else if (k == "-ir_tol") {
         o->ir_tol = std::atof(val(1).c_str());
       } else if (k == "-ir_iter") {

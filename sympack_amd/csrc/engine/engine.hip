@@ -275,6 +275,8 @@ int64_t Engine::device_bytes(int nrhs) const {
               vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) + vb(p.fused_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
+  b += (int64_t)n_ * 4 + 4;                                            // invp0, pivot flag
+  if (!p.fused_tasks.empty()) b += (int64_t)p.nsuper * 4;               // fused-factor load counters
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
   b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
   if (opt_.precision == 1)  // full CSR of A (row pointers, columns, sources, values), at most 2 nnz
@@ -353,7 +355,7 @@ void Engine::ensure_device() {
     // thread's graph capture
     HIP_OK(hipMemsetAsync(d_fcnt_, 0, sizeof(int) * (size_t)plan_.nsuper, s));
   }
-  if (plan_.df_ok) {
+  if (plan_.df_ok && use_df()) {  // switched off: nothing is uploaded
     d_df_fwd_ = dupload(plan_.df_fwd, s);
     d_df_bwd_ = dupload(plan_.df_bwd, s);
     d_df_fend_ = dupload(plan_.df_fwd_end, s);

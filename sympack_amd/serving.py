@@ -278,6 +278,13 @@ class SolverService:
                 with self._cv:
                     e.error = exc
                     e.state = "failed"
+                    # whatever DistributeMatrix / Factorize allocated before
+                    # the failure goes back before the budget counts it free
+                    if e.M is not None:
+                        try:
+                            e.M.ReleaseDevice()
+                        except Exception:  # noqa: BLE001 - the original error is the one reported
+                            pass
                     if e.resident:
                         self._resident_bytes -= e.bytes
                         e.resident = False

@@ -46,6 +46,36 @@ def test_options_from_args_like_run_sympack2D():
     assert o.orderingStr == "MMD" and o.relax.maxSize == 0
 
 
+@pytest.mark.parametrize("flag,want", [("FP32_IR", "FP32_IR"), ("FP32IR", "FP32_IR"), ("fp32", "FP32_IR"),
+                                       ("FP64", "FP64")])
+def test_precision_flag_spellings(flag, want):
+    assert sp.symPACKOptions.from_args(["-precision", flag]).precision == want
+
+
+def test_precision_flag_rejects_unknown_values():
+    with pytest.raises(ValueError, match="precision"):
+        sp.symPACKOptions.from_args(["-precision", "FP16"])
+
+
+def test_refine_code_outside_the_reordering_modes_is_rejected():
+    import ctypes
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.spk_create(ctypes.byref(h)) == 0
+    try:
+        o = _lib.SpkOptions()
+        lib.spk_default_options(ctypes.byref(o))
+        for good in (0, 10, 11, 20, 21, 30, 31):
+            o.refine = good
+            assert lib.spk_init(h, ctypes.byref(o)) == 0
+        for bad in (12, 40, 1, -5):
+            o.refine = bad
+            assert lib.spk_init(h, ctypes.byref(o)) == -2
+            assert b"refine" in lib.spk_last_error()
+    finally:
+        lib.spk_destroy(h)
+
+
 def test_rb_round_trip(tmp_path):
     n, cp, ri, v = M.diffusion_3d_7pt(5, seed=3)
     A = sp.DistSparseMatrix(n, cp, ri, v)

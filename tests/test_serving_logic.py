@@ -30,7 +30,9 @@ class DenseFactor:
     def DistributeMatrix(self, A):
         self.A = A.to_scipy_full().toarray()
         with DenseFactor.lock:
-            DenseFactor.live += 1
+            if not getattr(self, "held", False):
+                DenseFactor.live += 1
+            self.held = True
 
     def Factorize(self):
         if np.any(np.diag(self.A) <= 0):
@@ -46,9 +48,12 @@ class DenseFactor:
         return self.X
 
     def ReleaseDevice(self):
+        # idempotent, as symPACKMatrix2D.ReleaseDevice
         self.L = None
         with DenseFactor.lock:
-            DenseFactor.live -= 1
+            if getattr(self, "held", False):
+                DenseFactor.live -= 1
+            self.held = False
 
     def stats(self):
         return {"n": self.n}
@@ -109,6 +114,7 @@ def test_errors_reach_the_futures():
     v = v.copy()
     v[cp[3]] = -1.0
     bad = sp.DistSparseMatrix(n, cp, ri, v)
+    live0 = DenseFactor.live
     with SolverService(slots=2, matrix_factory=DenseFactor) as s:
         f = s.submit_factor("bad", bad)
         g = s.submit_solve("bad", np.ones(n))
@@ -118,6 +124,11 @@ def test_errors_reach_the_futures():
             g.result()
         with pytest.raises(KeyError):
             s.submit_solve("missing", np.ones(3))
+        # the failed factorization gave its device memory back, and the
+        # budget no longer counts it
+        s.drain()
+        assert DenseFactor.live == live0
+        assert s._resident_bytes == 0
 
 
 def test_request_stream_shape():
