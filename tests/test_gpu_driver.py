@@ -36,3 +36,19 @@ def test_driver_rejects_unknown_format(tmp_path):
     out = subprocess.run([DRIVER, "-in", str(tmp_path / "x"), "-inf", "MATRIX_MARKET"], capture_output=True,
                          text=True, timeout=60)
     assert out.returncode != 0
+
+
+ADAPTER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "adapter_c1")
+
+
+def test_cpp_adapter_full_sequence_on_c1():
+    """include/sympack_amd.hpp compiled into a C++ program that runs Init ->
+    SymbolicFactorization -> DistributeMatrix -> Factorize -> Solve ->
+    GetSolution on the 64x64 5-point Laplacian (MMD), LL^T and LDL^T, and
+    the not-HPD error."""
+    assert os.path.exists(ADAPTER), "adapter test not built"
+    out = subprocess.run([ADAPTER, "full"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    assert "adapter_c1: OK" in out.stdout
+    assert out.stdout.count("ok   ") >= 11
