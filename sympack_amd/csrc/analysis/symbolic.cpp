@@ -248,20 +248,22 @@ void relax_supernodes(const ETree& et, std::vector<Int>& cc, std::vector<Int>& s
 // symbolicFactorizationRelaxedDist (impl:613-938) on one rank: the row
 // structure of every supernode is the union of its children's structures
 // (beyond their own columns), its own columns, and the permuted matrix
-// entries below its first column, kept in a sorted linked list.
+// entries below its first column (of its first column only when relaxation
+// is off).  The reference builds the union in a sorted linked list, inserting
+// each new row by a walk from the head, which is quadratic in the width of
+// wide separators; the same sorted set is formed here with a marker array
+// and linear merges of the (sorted) child structures.  The size check
+// against the column count is the reference's.
 void symbolic_structure(const Graph& expanded, const Ordering& ord, const ETree& et,
                         const std::vector<Int>& cc, const std::vector<Int>& xsuper,
                         const std::vector<Int>& supMembership, bool first_col_only,
                         std::vector<Ptr>& xlindx, std::vector<Idx>& lindx) {
   Int size = expanded.n;
-  Graph pGraph = permute_graph(expanded, ord.invp);
   Int nsuper = (Int)xsuper.size() - 1;
-  const Idx tail = size + 1, head = 0;
-  std::vector<Idx> mrglnk(nsuper, 0), rchlnk(size + 1);
+  // children of every supernode, in the order the reference's merge list
+  // holds them (the latest linked first)
+  std::vector<Int> mrglnk(nsuper, 0);
   std::vector<Int> marker(size, 0);
-  Ptr nofsub = 0;
-  for (Int ksup = 1; ksup <= nsuper; ++ksup) nofsub += cc[xsuper[ksup - 1] - 1];
-  lindx.assign(nofsub, 0);
   xlindx.assign(nsuper + 1, 0);
   Ptr point = 1;
   for (Int ksup = 1; ksup <= nsuper; ++ksup) {
@@ -269,96 +271,72 @@ void symbolic_structure(const Graph& expanded, const Ordering& ord, const ETree&
     point += cc[xsuper[ksup - 1] - 1];
   }
   xlindx[nsuper] = point;
-  Ptr nzend = 0;
-  auto insert_sorted = [&](Idx newi, Ptr& knz, Int ksup) {
-    Idx nexti = head, i;
-    do {
-      i = nexti;
-      nexti = rchlnk[i];
-    } while (newi > nexti);
-    ++knz;
-    rchlnk[i] = newi;
-    rchlnk[newi] = nexti;
-    marker[newi - 1] = ksup;
-  };
+  lindx.assign(point - 1, 0);
+  // rows of the supernode past its own columns, kept sorted: children's
+  // structures are sorted, so each child's new rows are merged in linear
+  // time; the matrix entries are sorted on their own and merged last
+  std::vector<Idx> tail, tmp, merged;
   for (Int ksup = 1; ksup <= nsuper; ++ksup) {
-    Int fstcol = xsuper[ksup - 1], lstcol = xsuper[ksup] - 1;
-    Int width = lstcol - fstcol + 1;
-    Int length = cc[fstcol - 1];
-    Ptr knz = 0;
-    rchlnk[head] = tail;
-    Int jsup = mrglnk[ksup - 1];
-    if (jsup > 0) {
-      Int parentJ = ksup;
-      do {
-        Int jwidth = xsuper[jsup] - xsuper[jsup - 1];
-        Ptr jnzbeg = xlindx[jsup - 1] + jwidth, jnzend = xlindx[jsup] - 1;
-        if (parentJ == ksup) {
-          for (Ptr jptr = jnzend; jptr >= jnzbeg; --jptr) {
-            Idx newi = lindx[jptr - 1];
-            ++knz;
-            marker[newi - 1] = ksup;
-            rchlnk[newi] = rchlnk[head];
-            rchlnk[head] = newi;
-          }
-        } else {
-          Idx nexti = head;
-          for (Ptr jptr = jnzbeg; jptr <= jnzend; ++jptr) {
-            Idx newi = lindx[jptr - 1], i;
-            do {
-              i = nexti;
-              nexti = rchlnk[i];
-            } while (newi > nexti);
-            if (newi < nexti) {
-              ++knz;
-              rchlnk[i] = newi;
-              rchlnk[newi] = nexti;
-              marker[newi - 1] = ksup;
-              nexti = newi;
-            }
-          }
-        }
-        parentJ = jsup;
-        jsup = mrglnk[jsup - 1];
-      } while (jsup != 0 && knz < length);
-    }
-    if (knz < length) {
-      for (Int row = fstcol; row <= lstcol; ++row) {
-        Idx newi = row;
-        if (newi > fstcol && marker[newi - 1] != ksup) insert_sorted(newi, knz, ksup);
+    const Int fstcol = xsuper[ksup - 1], lstcol = xsuper[ksup] - 1;
+    const Int width = lstcol - fstcol + 1;
+    const Int length = cc[fstcol - 1];
+    for (Int row = fstcol; row <= lstcol; ++row) marker[row - 1] = ksup;
+    tail.clear();
+    auto merge_tmp = [&]() {
+      if (tmp.empty()) return;
+      if (tail.empty()) {
+        tail.swap(tmp);
+      } else {
+        merged.resize(tail.size() + tmp.size());
+        std::merge(tail.begin(), tail.end(), tmp.begin(), tmp.end(), merged.begin());
+        tail.swap(merged);
       }
+      tmp.clear();
+    };
+    for (Int jsup = mrglnk[ksup - 1]; jsup != 0 && width + (Int)tail.size() < length;
+         jsup = mrglnk[jsup - 1]) {
+      const Int jwidth = xsuper[jsup] - xsuper[jsup - 1];
+      for (Ptr jptr = xlindx[jsup - 1] + jwidth; jptr <= xlindx[jsup] - 1; ++jptr) {
+        const Idx r = lindx[jptr - 1];
+        if (marker[r - 1] != ksup) {
+          marker[r - 1] = ksup;
+          tmp.push_back(r);
+        }
+      }
+      merge_tmp();
+    }
+    if (width + (Int)tail.size() < length) {
+      // column col of the permuted matrix is column perm[col] of A, its rows
+      // mapped through invp (the order of the rows does not matter here)
       for (Int col = fstcol; col <= lstcol; ++col) {
-        for (Ptr kptr = pGraph.colptr[col - 1]; kptr <= pGraph.colptr[col] - 1; ++kptr) {
-          Idx newi = pGraph.rowind[kptr - 1];
-          if (newi > fstcol && marker[newi - 1] != ksup) insert_sorted(newi, knz, ksup);
+        const Int old = ord.perm[col - 1];
+        for (Ptr kptr = expanded.colptr[old - 1]; kptr <= expanded.colptr[old] - 1; ++kptr) {
+          const Idx newi = ord.invp[expanded.rowind[kptr - 1] - 1];
+          if (newi > (Idx)fstcol && marker[newi - 1] != ksup) {
+            marker[newi - 1] = ksup;
+            tmp.push_back(newi);
+          }
         }
         if (first_col_only) break;
       }
+      std::sort(tmp.begin(), tmp.end());
+      merge_tmp();
     }
-    if (rchlnk[head] != (Idx)fstcol) {
-      rchlnk[fstcol] = rchlnk[head];
-      rchlnk[head] = fstcol;
-      ++knz;
-    }
-    if (knz != length)
-      throw std::logic_error("symbolic factorization: structure size " + std::to_string(knz) +
+    if (width + (Int)tail.size() != length)
+      throw std::logic_error("symbolic factorization: structure size " + std::to_string(width + tail.size()) +
                              " != column count " + std::to_string(length) + " for supernode " +
                              std::to_string(ksup));
-    Ptr nzbeg = nzend + 1;
-    nzend += knz;
-    Idx i = head;
-    for (Ptr kptr = nzbeg; kptr <= nzend; ++kptr) {
-      i = rchlnk[i];
-      lindx[kptr - 1] = i;
-    }
+    Idx* out = lindx.data() + (xlindx[ksup - 1] - 1);
+    for (Int c = 0; c < width; ++c) out[c] = fstcol + c;
+    std::copy(tail.begin(), tail.end(), out + width);
     if (length > width) {
-      Idx pcol = et.PostParent(fstcol + width - 1 - 1);
-      Int psup = supMembership[pcol - 1];
+      // linked into the merge list of the supernode of its parent column
+      const Idx pcol = et.PostParent(fstcol + width - 1 - 1);
+      const Int psup = supMembership[pcol - 1];
       mrglnk[ksup - 1] = mrglnk[psup - 1];
       mrglnk[psup - 1] = ksup;
     }
   }
-  lindx.resize(nzend);
 }
 
 void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int baseval, int ordering,
@@ -420,9 +398,11 @@ void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int bas
 
   if (!have_sgraph) {
     // symPACKMatrix2D.hpp:2901-2937: etree on the permuted expanded graph.
-    std::vector<Int> backinvp = ord.invp;
-    Graph pg = permute_graph(graph, ord.invp);
-    et.construct_permuted(pg);
+    // the elimination tree of the permuted matrix, which is unique: Liu's
+    // algorithm on the unpermuted graph with rows mapped through invp
+    // (construct_serial) finds the same tree without building the permuted
+    // graph
+    et.construct_serial(graph, ord);
     et.postorder(ord);
     // getLColRowCount on the graph permuted by the postordered ordering.
     lcol_row_count(graph, et, ord, cc, out.rc);

@@ -271,7 +271,7 @@ void Engine::init(const EngineOptions& o) {
 int64_t Engine::device_bytes(int nrhs) const {
   auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
   const Plan& p = plan_;
-  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + vb(p.rel) + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
+  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + p.rel_size * 4 + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
               vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) + vb(p.fused_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
@@ -330,7 +330,25 @@ void Engine::ensure_device() {
   d_lindx0_ = dupload(plan_.lindx0, s);
   d_tcol_ = dupload(plan_.tcol, s);
   d_tmap_ = dupload(plan_.tmap, s);
-  d_rel_ = dupload(plan_.rel, s);
+  {
+    // relative indices of the Schur scatter, built on the device
+    RelGroup* groups = dupload(plan_.rel_groups, s);
+    d_rel_ = dalloc<int32_t>((size_t)std::max<int64_t>(1, plan_.rel_size));
+    int* bad = dalloc<int>(1);
+    HIP_OK(hipMemsetAsync(bad, 0, sizeof(int), s));
+    const int64_t ng = (int64_t)plan_.rel_groups.size();
+    if (ng > 0) {
+      dev::k_build_rel<<<(unsigned)std::min<int64_t>(ng, 65536), 256, 0, s>>>(
+          (const SnInfo*)d_sn_, d_lindx0_, groups, ng, d_rel_, bad);
+      HIP_OK(hipGetLastError());
+    }
+    int hbad = 0;
+    HIP_OK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    dfree(groups);
+    dfree(bad);
+    if (hbad) throw std::logic_error("symbolic structure is not closed under elimination");
+  }
   d_merges_ = dupload(plan_.merges, s);
   d_mmap_ = dupload(plan_.mmap, s);
   d_amap_ = dupload(plan_.amap, s);

@@ -110,6 +110,41 @@ __global__ void k_assemble(T* __restrict__ arena, const int64_t* __restrict__ am
   }
 }
 
+// Relative indices of the Schur scatter (the reference computes them per
+// update task, symPACKMatrix2D.hpp:1403-1439): for every group of rows of
+// supernode I that are columns of ancestor J, the position inside struct(J)
+// of each of I's off-diagonal rows from the group's first one on, found by
+// binary search in J's sorted structure.  One workgroup per group (grid
+// stride); *bad is set when a row is missing from struct(J), which a
+// symbolic structure closed under elimination never allows.
+__global__ __launch_bounds__(256) void k_build_rel(const SnInfo* __restrict__ sn,
+                                                   const int32_t* __restrict__ lindx0,
+                                                   const RelGroup* __restrict__ groups,
+                                                   int64_t ngroups, int32_t* __restrict__ rel,
+                                                   int* __restrict__ bad) {
+  for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+    const RelGroup g = groups[gi];
+    const SnInfo si = sn[g.I], sj = sn[g.J];
+    const int m = si.r - si.w;
+    const int32_t* __restrict__ rows = lindx0 + si.lx + si.w;
+    const int32_t* __restrict__ sJ = lindx0 + sj.lx;
+    for (int a = g.gb + (int)threadIdx.x; a < m; a += (int)blockDim.x) {
+      const int32_t row = rows[a];
+      int lo = 0, hi = sj.r;  // first position whose row is >= row
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sJ[mid] < row)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const bool found = lo < sj.r && sJ[lo] == row;
+      if (!found) atomicOr(bad, 1);
+      rel[g.base + (a - g.gb)] = found ? lo : 0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Scalar helpers for the pivots: hardware reciprocal / reciprocal square root
 // refined by Newton-Raphson (y <- y(3 - a y^2)/2, y <- y(2 - a y)), two steps

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <climits>
 #include <stdexcept>
+#include <exception>
+#include <thread>
 
 namespace spk {
 
@@ -52,12 +54,81 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   std::vector<int32_t> row_sup(n);
   for (Int i = 0; i < n; ++i) row_sup[i] = s.supMembership[i] - 1;
 
+  // Assembly map: caller entry (row,col) of the lower triangle goes to
+  // (max,min) of the permuted indices; later duplicates win.  Entries are
+  // bucketed by their permuted column (stable, in entry order); each
+  // supernode then marks the positions of its structure rows once, so every
+  // entry finds its row in O(1).  Two entries land on the same arena offset
+  // exactly when they share the permuted pair, i.e. the bucket and the row.
+  // It runs on its own thread beside the scatter maps and the merges (it
+  // writes only amap and reads the structure).
+  bool amap_error = false;
+  std::exception_ptr amap_exc;
+  std::thread amap_thread([&]() {
+    try {
+    std::vector<int32_t> apos(n, -1);
+    {
+      const Ptr nnz = colptr[n] - baseval;
+      p.amap.assign(nnz, -1);
+      std::vector<Ptr> bstart(n + 1, 0), order(nnz);
+      std::vector<int32_t> trow_of(nnz);
+      for (Int c = 0; c < n; ++c)
+        for (Ptr q = colptr[c] - baseval; q < colptr[c + 1] - baseval; ++q) {
+          const int32_t pr = s.ord.invp[rowind[q] - baseval] - 1, pc = s.ord.invp[c] - 1;
+          trow_of[q] = std::max(pr, pc);
+          ++bstart[std::min(pr, pc) + 1];
+        }
+      for (Int i = 0; i < n; ++i) bstart[i + 1] += bstart[i];
+      {
+        std::vector<Ptr> fill(bstart.begin(), bstart.end() - 1);
+        for (Int c = 0; c < n; ++c)
+          for (Ptr q = colptr[c] - baseval; q < colptr[c + 1] - baseval; ++q) {
+            const int32_t pr = s.ord.invp[rowind[q] - baseval] - 1, pc = s.ord.invp[c] - 1;
+            order[fill[std::min(pr, pc)]++] = q;
+          }
+      }
+      std::vector<int32_t> seen(n, -1);
+      std::vector<Ptr> last(n, -1);
+      for (Int J = 0; J < ns; ++J) {
+        const SnInfo& sj = p.sn[J];
+        const int32_t* stJ = &p.lindx0[sj.lx];
+        for (int32_t q = 0; q < sj.r; ++q) apos[stJ[q]] = q;
+        for (int32_t tcol = sj.fc; tcol < sj.fc + sj.w; ++tcol)
+          for (Ptr t = bstart[tcol]; t < bstart[tcol + 1]; ++t) {
+            const Ptr q = order[t];
+            const int32_t trow = trow_of[q];
+            const int32_t pp = apos[trow];
+            if (pp < 0) {
+              amap_error = true;
+              return;
+            }
+            p.amap[q] = sj.off + (int64_t)(tcol - sj.fc) * sj.r + pp;
+            if (seen[trow] == tcol) p.amap[last[trow]] = -1;  // an earlier entry of the same pair
+            seen[trow] = tcol;
+            last[trow] = q;
+          }
+        for (int32_t q = 0; q < sj.r; ++q) apos[stJ[q]] = -1;
+      }
+    }
+    } catch (...) {
+      amap_exc = std::current_exception();
+    }
+  });
+  struct Joiner {  // joins on every exit, also when the code below throws
+    std::thread& t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } amap_joiner{amap_thread};
+
   // Schur scatter maps.  For off-diagonal row b of I with target supernode J
   // (the supernode of column lindx[w+b]) the rows a >= first row of that
-  // target group are all in struct(J); rel stores their positions.
+  // target group are all in struct(J); their positions there (rel) are
+  // found on the device from one RelGroup per group.
   p.tcol.assign(upd, 0);
   p.tmap.assign(upd, 0);
-  p.rel.clear();
+  p.rel_groups.clear();
+  p.rel_size = 0;
   std::vector<int32_t> pos(n, -1);
   for (Int I = 0; I < ns; ++I) {
     const SnInfo& si = p.sn[I];
@@ -65,25 +136,15 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     int32_t m = si.r - si.w;
     int32_t b = 0;
     while (b < m) {
-      int32_t row = st[si.w + b];
-      int32_t J = row_sup[row];
+      int32_t J = row_sup[st[si.w + b]];
       const SnInfo& sj = p.sn[J];
       // group = consecutive off-diagonal rows falling in J's columns
       int32_t gb = b;
       while (b < m && row_sup[st[si.w + b]] == J) ++b;
-      int32_t ge = b;
-      // positions of struct(J)
-      const int32_t* stJ = &p.lindx0[sj.lx];
-      for (int32_t q = 0; q < sj.r; ++q) pos[stJ[q]] = q;
-      int64_t base = (int64_t)p.rel.size();
-      for (int32_t a = gb; a < m; ++a) {
-        int32_t pa = pos[st[si.w + a]];
-        if (pa < 0)
-          throw std::logic_error("symbolic structure is not closed under elimination");
-        p.rel.push_back(pa);
-      }
-      for (int32_t q = 0; q < sj.r; ++q) pos[stJ[q]] = -1;
-      for (int32_t bb = gb; bb < ge; ++bb) {
+      const int64_t base = p.rel_size;
+      p.rel_groups.push_back({base, I, J, gb, 0});
+      p.rel_size += m - gb;
+      for (int32_t bb = gb; bb < b; ++bb) {
         int32_t col = st[si.w + bb] - sj.fc;
         p.tcol[si.upd + bb] = sj.off + (int64_t)col * sj.r;
         p.tmap[si.upd + bb] = base - gb;
@@ -176,37 +237,9 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
   }
 
-  // Assembly map: caller entry (row,col) of the lower triangle goes to
-  // (max,min) of the permuted indices; later duplicates win.
-  {
-    Ptr nnz = colptr[n] - baseval;
-    p.amap.assign(nnz, -1);
-    // position of a permuted row inside struct(J): binary search
-    auto find_pos = [&](const SnInfo& sj, int32_t row) -> int32_t {
-      const int32_t* b = &p.lindx0[sj.lx];
-      const int32_t* e = b + sj.r;
-      const int32_t* it = std::lower_bound(b, e, row);
-      if (it == e || *it != row) return -1;
-      return (int32_t)(it - b);
-    };
-    for (Int c = 0; c < n; ++c) {
-      for (Ptr q = colptr[c] - baseval; q < colptr[c + 1] - baseval; ++q) {
-        int32_t r0 = rowind[q] - baseval;
-        int32_t pr = s.ord.invp[r0] - 1, pc = s.ord.invp[c] - 1;
-        int32_t trow = std::max(pr, pc), tcol = std::min(pr, pc);
-        const SnInfo& sj = p.sn[row_sup[tcol]];
-        int32_t pp = find_pos(sj, trow);
-        if (pp < 0) throw std::logic_error("matrix entry outside the symbolic structure");
-        p.amap[q] = sj.off + (int64_t)(tcol - sj.fc) * sj.r + pp;
-      }
-    }
-    // resolve duplicates: keep the last occurrence
-    std::vector<std::pair<int64_t, int64_t>> keyed(nnz);
-    for (Ptr q = 0; q < nnz; ++q) keyed[q] = {p.amap[q], q};
-    std::sort(keyed.begin(), keyed.end());
-    for (Ptr q = 0; q + 1 < nnz; ++q)
-      if (keyed[q].first == keyed[q + 1].first) p.amap[keyed[q].second] = -1;
-  }
+  amap_thread.join();
+  if (amap_exc) std::rethrow_exception(amap_exc);
+  if (amap_error) throw std::logic_error("matrix entry outside the symbolic structure");
 
   // inverse diagonal tiles: ceil(w/16) per supernode, contiguous, so block t
   // of step st is tile inv_base + 10 st + t

@@ -64,6 +64,16 @@ struct MergeInfo {         // mirrored on the device
   int32_t pad;
 };
 
+// One run of consecutive off-diagonal rows [gb, ge) of supernode I whose
+// rows are columns of the same ancestor J.  The relative indices of all of
+// I's off-diagonal rows a >= gb inside struct(J) are
+// rel[base + a - gb] (a in [gb, m_I)); they are filled on the device from
+// this table (binary search of each row in struct(J)).
+struct RelGroup {          // mirrored on the device
+  int64_t base;
+  int32_t I, J, gb, pad;
+};
+
 struct LaunchRange {
   int kind;          // 0 diagonal factor, 7 panel TRSM, 9 both fused, 1 in-panel update,
                      // 2 Schur update,
@@ -84,9 +94,11 @@ struct Plan {
   int32_t nlevels = 0;
   // Schur scatter maps (per off-diagonal row b of supernode I):
   // target address of element (a,b) = tcol[upd+b] + rel[tmap[upd+b] + a].
+  // rel (rel_size entries) exists on the device only, built from rel_groups.
   std::vector<int64_t> tcol;
   std::vector<int64_t> tmap;
-  std::vector<int32_t> rel;
+  std::vector<RelGroup> rel_groups;
+  int64_t rel_size = 0;
   // assembly: arena offset of every caller CSC entry (-1 = superseded duplicate)
   std::vector<int64_t> amap;
   std::vector<int32_t> perm0;      // perm0[new] = old, 0-based
