@@ -216,6 +216,74 @@ def serving_leg(n_requests, rate, precision, slots, seed=0):
     return rep
 
 
+def level_table(M, n, reps, title, cmd):
+    """Per-etree-level breakdown of the factorization and the solve from the
+    per-launch event trace (host-launched, every kernel kind timed): number
+    of supernodes, Schur-update tiles, algorithmic flops, block factor +
+    TRSM time, Schur time and rate, forward/backward solve time.  Returns
+    markdown text."""
+    import numpy as np
+
+    xs = np.asarray(M.GetSupernodalPartition())
+    xl, _ = M.GetStructure()
+    w = np.diff(xs).astype(np.float64)
+    r = np.diff(np.asarray(xl)).astype(np.float64)
+    m = r - w
+    lv = M.supernode_levels()
+    nl = int(lv.max()) + 1
+    schur_fl = np.bincount(lv, weights=w * m * (m + 1), minlength=nl)
+    diag_fl = np.bincount(lv, weights=w ** 3 / 3 + w * w * m, minlength=nl)
+    nsn = np.bincount(lv, minlength=nl)
+    M.reset_kernel_stats()
+    M.set_profile(True)
+    for _ in range(reps):
+        M.Factorize()
+        M.Solve(np.ones(n), 1)
+    M.set_profile(False)
+    kinds, grids, ms = M.launch_trace()
+    levels = M.launch_levels()
+    t = {k: np.zeros(nl) for k in ("blk", "schur", "fwd", "bwd")}
+    tiles = np.zeros(nl)
+    for k, g, x, L in zip(kinds, grids, ms, levels):
+        if L < 0:
+            continue
+        if k in (0, 1, 7, 9):
+            t["blk"][L] += x
+        elif k == 2:
+            t["schur"][L] += x
+            tiles[L] += g
+        elif k in (3, 4):
+            t["fwd"][L] += x
+        elif k in (5, 6):
+            t["bwd"][L] += x
+    for k in t:
+        t[k] /= reps
+    tiles /= reps
+    lines = ["# %s" % title, "", "Command (one MI355X, `R` = repository root):", "", "```", cmd, "```", "",
+             "Host-launched factorizations and nrhs = 1 solves with HIP events around every launch "
+             "(%d repetitions, averaged); times in microseconds per factorization / solve.  "
+             "Flops are algorithmic: Schur w m (m+1), block factor + TRSM w^3/3 + w^2 m per supernode "
+             "(w columns, m rows below the diagonal block)." % reps, "",
+             "| level | supernodes | Schur tiles | Schur GFLOP | block us | Schur us | Schur TFLOP/s | "
+             "block+Schur us | fwd solve us | bwd solve us |",
+             "|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    tot = {k: 0.0 for k in ("blk", "schur", "fwd", "bwd")}
+    for L in range(nl):
+        rate = schur_fl[L] / (t["schur"][L] * 1e-3) / 1e12 if t["schur"][L] > 0 else 0.0
+        lines.append("| %d | %d | %d | %.3f | %.1f | %.1f | %.1f | %.1f | %.1f | %.1f |" % (
+            L, nsn[L], tiles[L], schur_fl[L] / 1e9, t["blk"][L] * 1e3, t["schur"][L] * 1e3, rate,
+            (t["blk"][L] + t["schur"][L]) * 1e3, t["fwd"][L] * 1e3, t["bwd"][L] * 1e3))
+        for k in tot:
+            tot[k] += t[k][L]
+    lines.append("| all | %d | %d | %.3f | %.1f | %.1f | %.1f | %.1f | %.1f | %.1f |" % (
+        nsn.sum(), tiles.sum(), schur_fl.sum() / 1e9, tot["blk"] * 1e3, tot["schur"] * 1e3,
+        schur_fl.sum() / (tot["schur"] * 1e-3) / 1e12 if tot["schur"] > 0 else 0.0,
+        (tot["blk"] + tot["schur"]) * 1e3, tot["fwd"] * 1e3, tot["bwd"] * 1e3))
+    lines.append("")
+    lines.append("Block factor + TRSM flops in all: %.3f GFLOP." % (diag_fl.sum() / 1e9))
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,10 +298,18 @@ def main():
     ap.add_argument("--no-ldl", action="store_true",
                     help="skip the static LDL^T leg on the 80^3 shifted Laplacian")
     ap.add_argument("--ldl-k", type=int, default=80)
-    ap.add_argument("--serving", type=int, default=0, metavar="N",
-                    help="also run N requests of the serving mix (FP64 and FP32+IR)")
+    ap.add_argument("--serving", type=int, default=2000, metavar="N",
+                    help="requests of the serving stream run after the headline (FP64 and FP32+IR); "
+                         "0 skips it")
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--level-table", metavar="PATH",
+                    help="write the per-etree-level factor/solve breakdown (markdown) of the workload "
+                         "selected by --level-matrix and exit")
+    ap.add_argument("--level-matrix", default="diffusion48",
+                    choices=["diffusion48", "stencil27_60", "shifted80"],
+                    help="workload of --level-table: the 48^3 diffusion headline, the 60^3 27-point "
+                         "stencil or the 80^3 shifted Laplacian (LDL^T)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the factorization on one GPU only; "
@@ -244,6 +320,31 @@ def main():
     import torch
 
     import sympack_amd as sp
+
+    if args.level_table:
+        gens = {"diffusion48": ("3D 7-point diffusion 48^3 (n=110592), NDBOX, FP64 LL^T",
+                                lambda: sp.matrices.diffusion_3d_7pt(48, seed=0), "LL"),
+                "stencil27_60": ("3D 27-point SPD stencil 60^3 (n=216000), NDBOX, FP64 LL^T",
+                                 lambda: sp.matrices.stencil_3d_27pt(60, seed=0), "LL"),
+                "shifted80": ("shifted 3D 7-point Laplacian 80^3 (n=512000), NDBOX, FP64 static LDL^T",
+                              lambda: sp.matrices.shifted_laplacian_3d_7pt(80, 0.3), "LDL")}
+        title, gen, fact = gens[args.level_matrix]
+        n, cp, ri, v = gen()
+        A = sp.DistSparseMatrix(n, cp, ri, v)
+        rp = sp.RelaxationParameters()
+        rp.SetMaxSize(150)
+        M = sp.symPACKMatrix2D()
+        M.Init(sp.symPACKOptions(ordering="NDBOX", decomposition=fact, relax=rp))
+        M.SymbolicFactorization(A)
+        M.DistributeMatrix(A)
+        M.Factorize()
+        progress("level table: " + title)
+        cmd = "python3 $R/bench.py --level-table %s --level-matrix %s" % (args.level_table, args.level_matrix)
+        text = level_table(M, n, 5, "Per-level factor and solve times: " + title, cmd)
+        with open(args.level_table, "w") as f:
+            f.write(text)
+        print(text)
+        return
 
     progress("building %d^3 diffusion matrix" % args.k)
     A = build_problem(args.k)
@@ -357,8 +458,14 @@ def main():
     serving = None
     if args.serving:
         progress("serving leg")
-        serving = [serving_leg(args.serving, args.serving_rate, p, args.serving_slots)
-                   for p in ("FP64", "FP32_IR")]
+        serving = {}
+        for p in ("FP64", "FP32_IR"):
+            progress("serving leg %s: %d requests at %.0f/s" % (p, args.serving, args.serving_rate))
+            serving[p.lower()] = serving_leg(args.serving, args.serving_rate, p, args.serving_slots)
+        serving["workload"] = ("%d requests: 2D 5-point / 3D 7-point random-coefficient Laplacians, n "
+                               "log-uniform in [2e3, 2e5], 70%% follow-up solves with 1-32 right-hand sides, "
+                               "Poisson arrivals at %.0f/s, %d slots" %
+                               (args.serving, args.serving_rate, args.serving_slots))
 
     out = {
         "metric": "factor GFLOP/s; solves/s + p99 latency, FP32+IR vs FP64 at bwd err <=1e-12",

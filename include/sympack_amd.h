@@ -118,6 +118,14 @@ int spk_set_graphs(spk_handle h, int on);
 /* Per-launch records of the profiled calls since the last reset: kind, grid
  * size (workgroups) and event time; returns the number of records (<= max). */
 int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* ms, int max);
+/* Etree level (height above the leaves) of every record of
+ * spk_get_launch_trace, -1 for launches outside the level sequences; returns
+ * the number of records (<= max).  No reference counterpart. */
+int spk_get_launch_levels(spk_handle h, int32_t* levels, int max);
+/* Etree level of every supernode (0-based supernode index, 0 = leaves): the
+ * level whose launches factor it and solve with it.  No reference
+ * counterpart (the reference schedules tasks, not levels). */
+int spk_get_supernode_levels(spk_handle h, int32_t* level /* nsuper */);
 
 /* Library set-up, as the reference's C entry points (include/sympack.hpp:31-36,
  * src/global.cpp:114-185).  One rank: symPACK_Rank always returns 0. */

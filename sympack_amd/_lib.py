@@ -98,6 +98,8 @@ SIGNATURES = [
     ("spk_device_bytes", _I, [_P, _I, _PI64]),
     ("spk_release_device", _I, [_P]),
     ("spk_get_launch_trace", _I, [_P, _PI32, _PI64, _PD, _I]),
+    ("spk_get_launch_levels", _I, [_P, _PI32, _I]),
+    ("spk_get_supernode_levels", _I, [_P, _PI32]),
     ("spk_get_supernodal_partition", _I, [_P, _PI32]),
     ("spk_get_sup_membership", _I, [_P, _PI32]),
     ("spk_get_ordering", _I, [_P, _PI32, _PI32]),

@@ -407,6 +407,24 @@ int spk_get_launch_trace(spk_handle h, int32_t* kinds, int64_t* grids, double* m
   return cnt;
 }
 
+int spk_get_launch_levels(spk_handle h, int32_t* levels, int max) {
+  if (!h || !levels) return fail("null argument");
+  const auto& t = h->engine.trace;
+  int cnt = (int)std::min<size_t>(t.size(), (size_t)std::max(0, max));
+  for (int i = 0; i < cnt; ++i) levels[i] = t[i].level;
+  return cnt;
+}
+
+int spk_get_supernode_levels(spk_handle h, int32_t* level) {
+  if (!h || !level) return fail("null argument");
+  return guarded([&]() {
+    const spk::Plan& p = h->engine.plan();
+    if ((int)p.level.size() != p.nsuper) throw std::logic_error("no symbolic factorization");
+    std::copy(p.level.begin(), p.level.end(), level);
+    return 0;
+  });
+}
+
 int spk_reset_kernel_stats(spk_handle h) {
   if (!h) return fail("null handle");
   h->engine.reset_stats();

@@ -503,7 +503,7 @@ void Engine::assemble(T* arena) {
   HIP_OK(hipGetLastError());
 }
 
-void Engine::launch_begin(int kind, int64_t grid, hipStream_t st) {
+void Engine::launch_begin(int kind, int64_t grid, hipStream_t st, int level) {
   if (!st) st = stream_;
   if (capturing_)
     cap_kinds_.push_back(kind);
@@ -511,6 +511,7 @@ void Engine::launch_begin(int kind, int64_t grid, hipStream_t st) {
     kind_stats[kind].launches++;
   if (!opt_.profile || !(profile_mask_ & (1u << kind))) return;
   ev_grid_.push_back(grid);
+  ev_level_.push_back(level);
   while (ev_pool_.size() < ev_next_ + 2) {
     hipEvent_t e;
     HIP_OK(hipEventCreate(&e));
@@ -622,7 +623,7 @@ void Engine::collect_events() {
       float ms = 0.f;
       HIP_OK(hipEventElapsedTime(&ms, o.second.first, o.second.second));
       kind_stats[o.first].ms += ms;
-      if (trace.size() < 100000) trace.push_back({o.first, g->grids[i], ms});
+      if (trace.size() < 100000) trace.push_back({o.first, g->grids[i], ms, -1});
     }
   pending_graphs_.clear();
   for (size_t i = 0; i < ev_open_.size(); ++i) {
@@ -630,10 +631,12 @@ void Engine::collect_events() {
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, o.second.first, o.second.second));
     kind_stats[o.first].ms += ms;
-    if (trace.size() < 100000) trace.push_back({o.first, i < ev_grid_.size() ? ev_grid_[i] : 0, ms});
+    if (trace.size() < 100000)
+      trace.push_back({o.first, i < ev_grid_.size() ? ev_grid_[i] : 0, ms, i < ev_level_.size() ? ev_level_[i] : -1});
   }
   ev_open_.clear();
   ev_grid_.clear();
+  ev_level_.clear();
   ev_next_ = 0;
 }
 
@@ -730,7 +733,7 @@ void Engine::factor_sequence(T* arena) {
   for (const LaunchRange& r : plan_.factor_seq) {
     const unsigned g = (unsigned)r.count;
     hipStream_t st = stream_;
-    launch_begin(r.kind, r.count, st);
+    launch_begin(r.kind, r.count, st, r.level);
     switch (r.kind) {
       case 0: {
         const Task4* t = (const Task4*)d_tasks_[TA_POTRF] + r.first;
@@ -921,7 +924,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   const Task4* sm = (const Task4*)d_tasks_[TA_SOFFM];
   for (const LaunchRange& r : plan_.fwd_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind, r.count);
+    launch_begin(r.kind, r.count, nullptr, r.level);
     if (r.kind == 3) {
       if (r.aux > 0)
         launch_sdiag<T, false>(r.aux, g, sd + r.first, arena, Y, X, nrhs);
@@ -942,7 +945,7 @@ void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
   }
   for (const LaunchRange& r : plan_.bwd_seq) {
     const unsigned g = (unsigned)r.count;
-    launch_begin(r.kind, r.count);
+    launch_begin(r.kind, r.count, nullptr, r.level);
     if (r.kind == 5) {
       const dim3 grid((unsigned)r.count, (unsigned)((nrhs + 15) / 16));
       dev::k_soffm_bwd<T><<<grid, 256, 0, stream_>>>(sn, sm + r.first, arena, d_lindx0_, X, nrhs);

@@ -85,6 +85,7 @@ class Engine {
     int kind;
     int64_t grid;
     double ms;
+    int level;  // etree level of the launch (-1: outside the level sequences)
   };
   std::vector<LaunchRecord> trace;    // per launch of the last profiled calls
   void reset_stats();
@@ -120,7 +121,7 @@ class Engine {
   void precond(const double* r, double* z, int nrhs);
   void pcg_iteration(int nrhs);
   void pcg_restart(int nrhs);
-  void launch_begin(int kind, int64_t grid = 0, hipStream_t st = nullptr);
+  void launch_begin(int kind, int64_t grid = 0, hipStream_t st = nullptr, int level = -1);
   void launch_end(int kind, hipStream_t st = nullptr);
 
   // captured launch sequences
@@ -224,6 +225,7 @@ class Engine {
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
   std::vector<int64_t> ev_grid_;
+  std::vector<int> ev_level_;
   unsigned profile_mask_ = 0x3ffu;  // kinds whose launches get events
   size_t ev_next_ = 0;
   void collect_events();

@@ -432,14 +432,16 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   for (int L = 0; L < p.nlevels; ++L) {
     LaunchRange d{3, dfirst[L], (int64_t)by_level[L].size()};
     d.aux = dbucket[L];
+    d.level = L;
     p.fwd_seq.push_back(d);
-    if (mcount[L]) p.fwd_seq.push_back({4, mfirst[L], mcount[L]});
+    if (mcount[L]) p.fwd_seq.push_back({4, mfirst[L], mcount[L], 0, L});
   }
   // backward: levels top-down, off-diagonal gathers then diagonal solves.
   for (int L = p.nlevels - 1; L >= 0; --L) {
-    if (mcount[L]) p.bwd_seq.push_back({5, mfirst[L], mcount[L]});
+    if (mcount[L]) p.bwd_seq.push_back({5, mfirst[L], mcount[L], 0, L});
     LaunchRange d{6, dfirst[L], (int64_t)by_level[L].size()};
     d.aux = dbucket[L];
+    d.level = L;
     p.bwd_seq.push_back(d);
   }
 }

@@ -272,6 +272,19 @@ class symPACKMatrix2D:
                                                         _ptr(ms, ctypes.c_double), max_records), "trace")
         return k[:cnt], g[:cnt], ms[:cnt]
 
+    def launch_levels(self, max_records=100000):
+        """Etree level of every record of launch_trace() (-1 outside the level sequences)."""
+        lv = np.zeros(max_records, np.int32)
+        cnt = _lib.check(self._lib.spk_get_launch_levels(self._h, _ptr(lv, ctypes.c_int32), max_records),
+                         "launch levels")
+        return lv[:cnt]
+
+    def supernode_levels(self):
+        """Etree level of every supernode (0 = leaves), in supernode order."""
+        lv = np.zeros(self.TotalSupernodeCnt(), np.int32)
+        _lib.check(self._lib.spk_get_supernode_levels(self._h, _ptr(lv, ctypes.c_int32)), "supernode levels")
+        return lv
+
     def set_profile(self, on, kinds=None):
         """Time kernel launches with HIP events (all kinds, or the given kinds)."""
         if kinds is None:
