@@ -128,3 +128,54 @@ def backward_error(A, X, B):
     num = np.max(np.abs(R), axis=0)
     den = normA * np.max(np.abs(X), axis=0) + np.max(np.abs(B), axis=0)
     return float(np.max(num / den))
+
+
+def _blas_pointers():
+    """Addresses of dpotrf, dtrsm, dsyrk in scipy's LAPACK/BLAS (the
+    Fortran entry points its cython_blas / cython_lapack modules export)."""
+    import scipy.linalg.cython_blas as cb
+    import scipy.linalg.cython_lapack as cl
+    get = ctypes.pythonapi.PyCapsule_GetPointer
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    name = ctypes.pythonapi.PyCapsule_GetName
+    name.restype = ctypes.c_char_p
+    name.argtypes = [ctypes.py_object]
+
+    def ptr(cap):
+        return get(cap, name(cap))
+    return ptr(cl.__pyx_capi__["dpotrf"]), ptr(cb.__pyx_capi__["dtrsm"]), ptr(cb.__pyx_capi__["dsyrk"])
+
+
+def factor_blas(S, A, threads=1):
+    """BLAS-backed LL^T of the same structure (oracle_factor_blas): returns
+    (info, factor storage, seconds of the factorization proper, assembly
+    excluded).  threads: BLAS threads (the scatter runs on the calling thread)."""
+    import time
+    from threadpoolctl import threadpool_limits
+    L = lib()
+    P32 = ctypes.POINTER(ctypes.c_int32)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    PD = ctypes.POINTER(ctypes.c_double)
+    L.oracle_assemble.restype = ctypes.c_int
+    L.oracle_assemble.argtypes = [ctypes.c_int, ctypes.c_int, P32, P64, P32, P32, P64, P32, PD, PD]
+    L.oracle_factor_blas.restype = ctypes.c_int
+    L.oracle_factor_blas.argtypes = [ctypes.c_int, ctypes.c_int, P32, P64, P32, PD, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+    size = L.oracle_factor_size(S.n, S.nsuper, _p(S.xsuper, ctypes.c_int32), _p(S.xlindx, ctypes.c_int64))
+    F = np.zeros(size, np.float64)
+    cp = np.ascontiguousarray(A.colptr - A.baseval, np.int64)
+    ri = np.ascontiguousarray(A.rowind - A.baseval, np.int32)
+    rc = L.oracle_assemble(S.n, S.nsuper, _p(S.xsuper, ctypes.c_int32), _p(S.xlindx, ctypes.c_int64),
+                           _p(S.lindx, ctypes.c_int32), _p(S.perm, ctypes.c_int32), _p(cp, ctypes.c_int64),
+                           _p(ri, ctypes.c_int32), _p(np.ascontiguousarray(A.nzval), ctypes.c_double),
+                           _p(F, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("assembly failed")
+    potrf, trsm, syrk = _blas_pointers()
+    with threadpool_limits(limits=threads, user_api="blas"):
+        t0 = time.perf_counter()
+        info = L.oracle_factor_blas(S.n, S.nsuper, _p(S.xsuper, ctypes.c_int32), _p(S.xlindx, ctypes.c_int64),
+                                    _p(S.lindx, ctypes.c_int32), _p(F, ctypes.c_double), potrf, trsm, syrk)
+        dt = time.perf_counter() - t0
+    return info, F, dt

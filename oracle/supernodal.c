@@ -299,3 +299,108 @@ int oracle_solve(int n, int nsuper, const int32_t* xsuper, const int64_t* xlindx
   release(&P);
   return 0;
 }
+
+/*
+ * BLAS-backed LL^T with the same storage, used as the CPU baseline: the
+ * reference's per-supernode LAPACK/BLAS calls on the row-major cells
+ * (Appendix of the call translation: column-major view U = L^T), with the
+ * BLAS entry points passed in by the caller (scipy's OpenBLAS):
+ *   factorize  Potrf('U', w, A, w)                    symPACKMatrix2D.hpp:1063,
+ *                                                     src/lapack.cpp:293-306
+ *   trsm       Trsm('L','U','T','N', w, m, 1, A, w, B, w)          :1120
+ *   update     one Syrk('L','T', m, w, 1, B, w, 0, U, m) per supernode for
+ *              all its target cells (the reference issues one Syrk/Gemm per
+ *              target cell, :1288-1341), then the scatter through the
+ *              relative row positions (:1393-1466), on the calling thread
+ *              (the BLAS calls run on the BLAS library's own threads).
+ * L holds the assembled matrix (oracle_assemble) on entry and the factor on
+ * return.  Returns 0, the first failing pivot column (1-based), or -1.
+ */
+typedef void (*potrf_fn)(char*, int*, double*, int*, int*);
+typedef void (*trsm_fn)(char*, char*, char*, char*, int*, int*, double*, double*, int*, double*, int*);
+typedef void (*syrk_fn)(char*, char*, int*, int*, double*, double*, int*, double*, double*, int*);
+
+int oracle_assemble(int n, int nsuper, const int32_t* xsuper, const int64_t* xlindx,
+                    const int32_t* lindx, const int32_t* perm, const int64_t* colptr,
+                    const int32_t* rowind, const double* nzval, double* L) {
+  plan_t P;
+  build(&P, n, nsuper, xsuper, xlindx, lindx, 0);
+  int rc = 0;
+  int32_t* invp = (int32_t*)malloc(sizeof(int32_t) * n);
+  for (int i = 0; i < n; ++i) invp[perm[i] - 1] = i + 1;
+  memset(L, 0, sizeof(double) * P.off[nsuper]);
+  for (int c = 0; c < n && rc == 0; ++c)
+    for (int64_t q = colptr[c]; q < colptr[c + 1]; ++q) {
+      int pr = invp[rowind[q]], pc = invp[c];
+      int trow = pr > pc ? pr : pc, tcol = pr > pc ? pc : pr;
+      int J = P.sup[tcol - 1];
+      int w = xsuper[J + 1] - xsuper[J];
+      int pos = find_row(&P, J, trow);
+      if (pos < 0) {
+        rc = -1;
+        break;
+      }
+      L[P.off[J] + (int64_t)pos * w + (tcol - xsuper[J])] = nzval[q];
+    }
+  free(invp);
+  release(&P);
+  return rc;
+}
+
+int oracle_factor_blas(int n, int nsuper, const int32_t* xsuper, const int64_t* xlindx,
+                       const int32_t* lindx, double* L, void* potrf_p, void* trsm_p, void* syrk_p) {
+  potrf_fn potrf = (potrf_fn)potrf_p;
+  trsm_fn trsm = (trsm_fn)trsm_p;
+  syrk_fn syrk = (syrk_fn)syrk_p;
+  plan_t P;
+  build(&P, n, nsuper, xsuper, xlindx, lindx, 0);
+  int info = 0;
+  int* map = (int*)malloc(sizeof(int) * (n + 1));
+  double* U = NULL;
+  size_t ucap = 0;
+  char cU = 'U', cL = 'L', cT = 'T', cN = 'N';
+  double one = 1.0, zero = 0.0;
+  for (int I = 0; I < nsuper && info == 0; ++I) {
+    int w = xsuper[I + 1] - xsuper[I];
+    const int r = (int)(xlindx[I + 1] - xlindx[I]);
+    int m = r - w;
+    const int32_t* st = lindx + (xlindx[I] - 1);
+    double* A = L + P.off[I];
+    int pinfo = 0;
+    potrf(&cU, &w, A, &w, &pinfo);
+    if (pinfo > 0) {
+      info = xsuper[I] + pinfo - 1;
+      break;
+    }
+    if (m <= 0) continue;
+    double* B = A + (int64_t)w * w;
+    trsm(&cL, &cU, &cT, &cN, &w, &m, &one, A, &w, B, &w);
+    if ((size_t)m * m > ucap) {
+      free(U);
+      ucap = (size_t)m * m;
+      U = (double*)malloc(sizeof(double) * ucap);
+    }
+    syrk(&cL, &cT, &m, &w, &one, B, &w, &zero, U, &m);
+    int b0 = 0;
+    while (b0 < m) {
+      const int J = P.sup[st[w + b0] - 1];
+      int b1 = b0;
+      while (b1 < m && P.sup[st[w + b1] - 1] == J) ++b1;
+      const int wJ = xsuper[J + 1] - xsuper[J];
+      const int rJ = (int)(xlindx[J + 1] - xlindx[J]);
+      const int32_t* stJ = lindx + (xlindx[J] - 1);
+      for (int q = 0; q < rJ; ++q) map[stJ[q]] = q;
+      double* T = L + P.off[J];
+      for (int b = b0; b < b1; ++b) {
+        const int col = st[w + b] - xsuper[J];
+        const double* ub = U + (size_t)b * m;
+        for (int a = b; a < m; ++a) T[(int64_t)map[st[w + a]] * wJ + col] -= ub[a];
+      }
+      b0 = b1;
+    }
+  }
+  free(U);
+  free(map);
+  release(&P);
+  return info;
+}

@@ -89,3 +89,29 @@ def test_not_spd_detected():
     m.SymbolicFactorization(A)
     info, F = onum.factor(onum.Structure.of(m), A)
     assert info > 0
+
+
+@pytest.mark.parametrize("k,threads", [(10, 1), (16, 4)])
+def test_blas_baseline_matches_scalar_oracle(k, threads):
+    """The BLAS-backed CPU baseline (per-supernode dpotrf/dtrsm/dsyrk) gives
+    the scalar oracle's factor, and its solve recovers x_true."""
+    n, cp, ri, v = M.diffusion_3d_7pt(k, seed=k)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    m = sp.symPACKMatrix2D()
+    m.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp))
+    m.SymbolicFactorization(A)
+    S = onum.Structure.of(m)
+    info, F = onum.factor(S, A)
+    info2, F2, dt = onum.factor_blas(S, A, threads)
+    assert info == info2 == 0 and dt > 0
+    # compare the lower triangles of the diagonal blocks and the panels below
+    for P1, P2 in zip(onum.panels(S, F), onum.panels(S, F2)):
+        w = P1.shape[1]
+        mask = np.ones_like(P1, bool)
+        mask[:w, :w] = np.tril(np.ones((w, w), bool))
+        assert np.allclose(P1[mask], P2[mask], rtol=1e-12, atol=1e-13 * np.abs(P1).max())
+    xt = np.random.default_rng(k).uniform(-1, 1, n)
+    x = onum.solve(S, F2, A.matvec(xt))
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-10
