@@ -252,9 +252,9 @@ def level_table(M, n, reps, title, cmd):
         elif k == 2:
             t["schur"][L] += x
             tiles[L] += g
-        elif k in (3, 4):
+        elif k in (3, 4, 10):
             t["fwd"][L] += x
-        elif k in (5, 6):
+        elif k in (5, 6, 11):
             t["bwd"][L] += x
     for k in t:
         t[k] /= reps
@@ -390,7 +390,7 @@ def main():
         sol_ms.append((d - c) * 1e3)
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
-    launches = {k: M.kernel_stats(k)[0] for k in range(10)}
+    launches = {k: M.kernel_stats(k)[0] for k in range(13)}
 
     # timed region 2: K factorizations launched from the host with HIP events
     # around every Schur-update launch (kind 2) on the library's stream, for
@@ -404,7 +404,7 @@ def main():
         M.Factorize()
         prof_ms.append((time.perf_counter() - a) * 1e3)
     M.set_profile(False)
-    kinds = {k: M.kernel_stats(k) for k in range(10)}
+    kinds = {k: M.kernel_stats(k) for k in range(13)}
 
     # solve latency distribution (factor resident, repeated solves)
     lat = []
@@ -513,7 +513,9 @@ def main():
                               for k, name in [(0, "potrf"), (7, "trsm"), (9, "potrf_trsm_fused"),
                                               (1, "inpanel_update"),
                                               (2, "schur_update"), (3, "fwd_diag"), (4, "fwd_offdiag"),
-                                              (5, "bwd_offdiag"), (6, "bwd_diag")]},
+                                              (5, "bwd_offdiag"), (6, "bwd_diag"),
+                                              (10, "fwd_partitioned_inverse"), (11, "bwd_partitioned_inverse"),
+                                              (12, "partitioned_inverse_prep")]},
         "roofline": {"kernel": "k_update<double,1> (Schur update, FP64 MFMA + atomic scatter)",
                      "timing": "HIP events around each Schur-update launch, %d host-launched "
                                "factorizations after the graph-replayed throughput loop" % args.steps,

@@ -73,6 +73,12 @@ typedef struct spk_options {
                          symPACKMatrix2D.hpp:3090-3111): 0 = NO (default), or
                          10*ordflag + altflag: 10, 11, 20, 21, 30, 31 for SET10 .. SET31
                          (SET = 31) */
+  int solve_inverse;  /* 1 (default): where every supernode is at most 160 columns wide, the
+                         solve uses the partitioned inverse G_I = [I; L_off] L_II^{-1} of each
+                         supernode (prepared once per factorization), one launch per etree level
+                         and sweep; for LL^T and for the FP32 factor under refinement, not for
+                         FP64 static LDL^T (explicit inverses of its indefinite diagonal blocks
+                         lose accuracy).  0: block substitution, four launches per level */
 } spk_options;
 
 typedef struct spk_stats {
@@ -96,7 +102,8 @@ typedef struct spk_stats {
 /* Per-kernel-kind event timings (profile mode).  kind: 0 diagonal factor,
  * 7 panel TRSM, 9 fused diagonal factor + panel TRSM, 1 in-panel update,
  * 2 Schur update, 3 forward diagonal, 4 forward off-diagonal, 5 backward
- * off-diagonal, 6 backward diagonal. */
+ * off-diagonal, 6 backward diagonal, 10 forward / 11 backward
+ * partitioned-inverse solve, 12 its preparation. */
 int spk_get_kernel_stats(spk_handle h, int kind, int64_t* launches, double* ms);
 int spk_reset_kernel_stats(spk_handle h);
 /* Switch per-launch event timing on/off after spk_init (no reference
@@ -202,6 +209,21 @@ int spk_read_rb(const char* filename, int32_t* n, int64_t* nnz);
 int spk_read_rb_data(const char* filename, int64_t* colptr, int32_t* rowind, double* nzval);
 int spk_write_rb(const char* filename, int32_t n, const int64_t* colptr, const int32_t* rowind,
                  const double* nzval, const char* title);
+
+/* ReadMatrix (utility.hpp:1693-1716) on one rank: format "CSC" (binary,
+ * ParaReadDistSparseMatrix, utility.hpp:1037-1265), "HB" / "RB" /
+ * "HARWELL_BOEING" (ReadHB_PARA) or "matrix" (text,
+ * ReadDistSparseMatrixFormatted, utility.hpp:1486-1688); any other name is
+ * "Unknown matrix format." (status -2).  Returns the lower triangle, 1-based,
+ * rows sorted per column for CSC and HB/RB and in file order for "matrix"
+ * (a full symmetric "matrix" file is reduced to its lower triangle). */
+int spk_read_matrix(const char* filename, const char* format, int32_t* n, int64_t* nnz);
+int spk_read_matrix_data(const char* filename, const char* format, int64_t* colptr, int32_t* rowind,
+                         double* nzval);
+/* Writes a lower-triangular CSC (any base) in one of the formats above (no
+ * reference counterpart: the reference only reads). */
+int spk_write_matrix(const char* filename, const char* format, int32_t n, const int64_t* colptr,
+                     const int32_t* rowind, const double* nzval);
 
 const char* spk_last_error(void);
 

@@ -36,6 +36,7 @@ class SpkOptions(ctypes.Structure):
         ("merge_updates", ctypes.c_int),
         ("solve_dataflow", ctypes.c_int),
         ("refine", ctypes.c_int),
+        ("solve_inverse", ctypes.c_int),
     ]
 
 
@@ -117,6 +118,9 @@ SIGNATURES = [
     ("spk_read_rb", _I, [ctypes.c_char_p, _PI32, _PI64]),
     ("spk_read_rb_data", _I, [ctypes.c_char_p, _PI64, _PI32, _PD]),
     ("spk_write_rb", _I, [ctypes.c_char_p, _I32, _PI64, _PI32, _PD, ctypes.c_char_p]),
+    ("spk_read_matrix", _I, [ctypes.c_char_p, ctypes.c_char_p, _PI32, _PI64]),
+    ("spk_read_matrix_data", _I, [ctypes.c_char_p, ctypes.c_char_p, _PI64, _PI32, _PD]),
+    ("spk_write_matrix", _I, [ctypes.c_char_p, ctypes.c_char_p, _I32, _PI64, _PI32, _PD]),
     ("spk_last_error", ctypes.c_char_p, []),
 ]
 

@@ -168,6 +168,15 @@ struct CscMatrix {
   std::vector<double> nzval;
 };
 void read_rb(const std::string& filename, CscMatrix& out);
+// Binary CSC (ParaReadDistSparseMatrix, utility.hpp:1037-1265) and the text
+// "matrix" format (ReadDistSparseMatrixFormatted, utility.hpp:1486-1688), and
+// ReadMatrix's dispatch by format name (utility.hpp:1702-1716): "CSC",
+// "HB" / "RB" / "HARWELL_BOEING", "matrix"; other names raise logic_error.
+void read_csc_binary(const std::string& filename, CscMatrix& out);
+void write_csc_binary(const std::string& filename, const CscMatrix& a);
+void read_matrix_formatted(const std::string& filename, CscMatrix& out);
+void write_matrix_formatted(const std::string& filename, const CscMatrix& a);
+void read_matrix(const std::string& filename, const std::string& format, CscMatrix& out);
 void write_rb(const std::string& filename, const CscMatrix& a, const std::string& title,
               const std::string& key);
 

@@ -82,6 +82,7 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   if (o.precision != SPK_PREC_FP64 && o.precision != SPK_PREC_FP32_IR)
     throw std::logic_error("precision must be SPK_PREC_FP64 or SPK_PREC_FP32_IR");
   e.refine = o.refine;
+  e.solve_inverse = o.solve_inverse;
   return e;
 }
 }  // namespace
@@ -139,6 +140,7 @@ void spk_default_options(spk_options* o) {
   o->merge_updates = 1;
   o->solve_dataflow = 0;
   o->refine = 0;
+  o->solve_inverse = 1;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -214,6 +216,8 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         else if (v.compare(0, 4, "TSPB") == 0)
           throw std::logic_error("-refine TSPB is not supported; use SET, SET10 .. SET31");
         else o->refine = 0;
+      } else if (k == "-solve_inverse") {
+        o->solve_inverse = std::atoi(val(1).c_str()) != 0;
       } else if (k == "-dataflow") {
         o->solve_dataflow = std::atoi(val(1).c_str()) != 0;
       }
@@ -342,7 +346,8 @@ int spk_get_stats(spk_handle h, spk_stats* st) {
     st->schur_entries = p.schur_entries;
     const auto& ks = h->engine.kind_stats;
     st->launches_factor = ks[0].launches + ks[1].launches + ks[2].launches + ks[7].launches + ks[9].launches;
-    st->launches_solve = ks[3].launches + ks[4].launches + ks[5].launches + ks[6].launches;
+    st->launches_solve = ks[3].launches + ks[4].launches + ks[5].launches + ks[6].launches +
+                         ks[10].launches + ks[11].launches + ks[12].launches;
     st->factor_ms = h->engine.last_factor_ms;
     st->solve_ms = h->engine.last_solve_ms;
     st->ir_iterations = h->engine.last_report.iterations;
@@ -619,6 +624,57 @@ int spk_write_rb(const char* filename, int32_t n, const int64_t* colptr, const i
       for (auto& r : a.rowind) r += 1;
     a.nzval.assign(nzval, nzval + nnz);
     spk::write_rb(filename, a, title ? title : "sympack_amd", "SPKAMD");
+    return 0;
+  });
+}
+
+int spk_read_matrix(const char* filename, const char* format, int32_t* n, int64_t* nnz) {
+  if (!filename || !format || !n || !nnz) return fail("null argument");
+  return guarded([&]() {
+    spk::CscMatrix a;
+    spk::read_matrix(filename, format, a);
+    *n = a.n;
+    *nnz = (int64_t)a.rowind.size();
+    return 0;
+  });
+}
+
+int spk_read_matrix_data(const char* filename, const char* format, int64_t* colptr, int32_t* rowind,
+                         double* nzval) {
+  if (!filename || !format || !colptr || !rowind || !nzval) return fail("null argument");
+  return guarded([&]() {
+    spk::CscMatrix a;
+    spk::read_matrix(filename, format, a);
+    std::memcpy(colptr, a.colptr.data(), a.colptr.size() * sizeof(int64_t));
+    std::memcpy(rowind, a.rowind.data(), a.rowind.size() * sizeof(int32_t));
+    std::memcpy(nzval, a.nzval.data(), a.nzval.size() * sizeof(double));
+    return 0;
+  });
+}
+
+int spk_write_matrix(const char* filename, const char* format, int32_t n, const int64_t* colptr,
+                     const int32_t* rowind, const double* nzval) {
+  if (!filename || !format || !colptr || !rowind || !nzval) return fail("null argument");
+  return guarded([&]() {
+    spk::CscMatrix a;
+    a.n = n;
+    const int64_t base = colptr[0];
+    const int64_t nnz = colptr[n] - base;
+    a.colptr.resize(n + 1);
+    for (int32_t i = 0; i <= n; ++i) a.colptr[i] = colptr[i] - base + 1;
+    a.rowind.assign(rowind, rowind + nnz);
+    if (base == 0)
+      for (auto& r : a.rowind) r += 1;
+    a.nzval.assign(nzval, nzval + nnz);
+    const std::string f = format;
+    if (f == "CSC")
+      spk::write_csc_binary(filename, a);
+    else if (f == "matrix")
+      spk::write_matrix_formatted(filename, a);
+    else if (f == "HB" || f == "RB" || f == "HARWELL_BOEING")
+      spk::write_rb(filename, a, "sympack_amd", "SPKAMD");
+    else
+      throw std::logic_error("Unknown matrix format.");
     return 0;
   });
 }

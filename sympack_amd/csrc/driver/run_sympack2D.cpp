@@ -1,5 +1,6 @@
 // run_sympack2D-compatible driver (driver/run_sympack2D.cpp:22-140 of the
-// reference): reads a Harwell-Boeing / Rutherford-Boeing lower triangle,
+// reference): reads a Harwell-Boeing / Rutherford-Boeing, binary CSC or
+// text "matrix" lower triangle (ReadMatrix, utility.hpp:1693-1716),
 // builds b = A x_true with the reference's alternating +-1 x_true
 // (driver/utils.hpp:9-91), runs Init / SymbolicFactorization /
 // DistributeMatrix / Factorize / Solve / GetSolution on one GPU and prints the
@@ -38,11 +39,13 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[i], "-nrhs") && i + 1 < argc) nrhs = std::atoi(argv[i + 1]);
   }
   if (filename.empty()) {
-    std::fprintf(stderr, "usage: %s -in <file.rb> [-inf RB|HB] [-ordering MMD|...] [-nrhs k]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s -in <file> [-inf HARWELL_BOEING|RB|HB|CSC|matrix] [-ordering MMD|...] [-nrhs k]\n",
+                 argv[0]);
     return 1;
   }
-  if (informat != "HARWELL_BOEING" && informat != "RB" && informat != "HB") {
-    std::fprintf(stderr, "Unknown input format %s\n", informat.c_str());
+  if (informat != "HARWELL_BOEING" && informat != "RB" && informat != "HB" && informat != "CSC" &&
+      informat != "matrix") {
+    std::fprintf(stderr, "Unknown matrix format. (%s)\n", informat.c_str());
     return 1;
   }
   spk_options opt;
@@ -51,11 +54,12 @@ int main(int argc, char** argv) {
 
   int32_t n = 0;
   int64_t nnz = 0;
-  check(spk_read_rb(filename.c_str(), &n, &nnz), "ReadMatrix");
+  check(spk_read_matrix(filename.c_str(), informat.c_str(), &n, &nnz), "ReadMatrix");
   std::vector<int64_t> colptr(n + 1);
   std::vector<int32_t> rowind(nnz);
   std::vector<double> val(nnz);
-  check(spk_read_rb_data(filename.c_str(), colptr.data(), rowind.data(), val.data()), "ReadMatrix");
+  check(spk_read_matrix_data(filename.c_str(), informat.c_str(), colptr.data(), rowind.data(), val.data()),
+        "ReadMatrix");
 
   // b = A x_true with x_true alternating +1/-1 (utils.hpp:22-28)
   std::vector<double> xtrue((size_t)n * nrhs), rhs((size_t)n * nrhs, 0.0);

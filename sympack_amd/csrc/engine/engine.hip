@@ -10,6 +10,8 @@
 #include <cmath>
 #include <stdexcept>
 
+#include <mutex>
+
 #include "kernels.hip"
 
 namespace spk {
@@ -49,7 +51,7 @@ inline unsigned grid_for(int64_t n, int block = 256) {
 }
 // k_gather_rhs / k_scatter_sol: one workgroup per 64 rows
 inline unsigned grid_rows64(int n) { return (unsigned)std::max(1, (n + 63) / 64); }
-enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED };
+enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED, TA_GINV };
 }  // namespace
 
 namespace {
@@ -182,6 +184,64 @@ __global__ __launch_bounds__(256) void k_csr_mv(const int64_t* __restrict__ rowp
     }
   }
 }
+// GMRES (refinement of the FP32 static LDL^T factor): column-major n x nrhs
+// vectors, one Krylov space per right-hand side.
+// part[((i nrhs + j) kRedBlocks + b)] = partial <V_i(:,j), w(:,j)> over block b's rows
+__global__ void k_multidot_part(const double* __restrict__ V, int64_t vstride, const double* __restrict__ w,
+                                int n, int nrhs, double* __restrict__ part) {
+  const int j = blockIdx.y, i = blockIdx.z;
+  const int64_t per = ((int64_t)n + kRedBlocks - 1) / kRedBlocks;
+  const int64_t i0 = (int64_t)blockIdx.x * per, i1 = std::min<int64_t>(n, i0 + per);
+  const double* __restrict__ v = V + (int64_t)i * vstride + (int64_t)j * n;
+  const double* __restrict__ x = w + (int64_t)j * n;
+  double acc = 0.0, unused = 0.0;
+  for (int64_t r = i0 + threadIdx.x; r < i1; r += 256) acc += v[r] * x[r];
+  block_reduce2(acc, unused, false);
+  if (threadIdx.x == 0) part[((int64_t)i * nrhs + j) * kRedBlocks + blockIdx.x] = acc;
+}
+
+// out[i nrhs + j] = sum of the partials (fixed order)
+__global__ void k_multidot_fin(const double* __restrict__ part, double* __restrict__ out) {
+  const int64_t q = blockIdx.x;  // i nrhs + j
+  const int t = threadIdx.x;
+  double acc = t < kRedBlocks ? part[q * kRedBlocks + t] : 0.0, unused = 0.0;
+  block_reduce2(acc, unused, false);
+  if (t == 0) out[q] = acc;
+}
+
+// w(:,j) -= sum_{i < kv} h[i nrhs + j] V_i(:,j)
+__global__ void k_gs_update(double* __restrict__ w, const double* __restrict__ V, int64_t vstride,
+                            const double* __restrict__ h, int kv, int n, int nrhs) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n);
+    double acc = 0.0;
+    for (int i = 0; i < kv; ++i) acc += h[i * nrhs + j] * V[(int64_t)i * vstride + idx];
+    w[idx] -= acc;
+  }
+}
+
+// dst(:,j) = src(:,j) / sqrt(nrm2[j]) (0 where nrm2[j] <= 0)
+__global__ void k_scale_cols(double* __restrict__ dst, const double* __restrict__ src,
+                             const double* __restrict__ nrm2, int n, int nrhs) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const double q = nrm2[idx / n];
+    dst[idx] = q > 0.0 ? src[idx] / sqrt(q) : 0.0;
+  }
+}
+
+// x(:,j) += sum_{i < kv} y[i nrhs + j] Z_i(:,j)
+__global__ void k_add_combo(double* __restrict__ x, const double* __restrict__ Z, int64_t zstride,
+                            const double* __restrict__ y, int kv, int n, int nrhs) {
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)n * nrhs;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n);
+    double acc = 0.0;
+    for (int i = 0; i < kv; ++i) acc += y[i * nrhs + j] * Z[(int64_t)i * zstride + idx];
+    x[idx] += acc;
+  }
+}
 }  // namespace
 
 // per-RHS max |r| and max |x| into out[2j], out[2j+1]
@@ -201,7 +261,7 @@ void Engine::csr_mv(const double* x, const double* b, double* out, int nrhs) {
   k_csr_mv<<<grid_for(n_), 256, 0, stream_>>>(d_crow_, d_ccol_, d_cval_, x, b, out, n_, nrhs);
 }
 
-Engine::Engine() { kind_stats.resize(10); }
+Engine::Engine() { kind_stats.resize(13); }
 
 Engine::~Engine() {
   try {
@@ -249,8 +309,11 @@ void Engine::free_device() {
   dfree(d_arena64_);
   dfree(d_arena32_);
   dfree(d_dinv_);
+  dfree(d_garena_);
+  ginv_ready_ = false;
   dfree(d_Y_);
   dfree(d_X_);
+  dfree(d_Xf_);
   dfree(d_b_);
   dfree(d_x_);
   dfree(d_r_);
@@ -260,6 +323,9 @@ void Engine::free_device() {
   dfree(d_p_);
   dfree(d_q_);
   dfree(d_s_);
+  dfree(d_gmV_);
+  dfree(d_gmh_);
+  gm_cap_ = gm_hcap_ = 0;
   cap_nrhs_ = 0;
 }
 
@@ -279,10 +345,11 @@ int64_t Engine::device_bytes(int nrhs) const {
   if (!p.fused_tasks.empty()) b += (int64_t)p.nsuper * 4;               // fused-factor load counters
   const int64_t elem = opt_.precision == 0 ? 8 : 4;
   b += p.arena_elems * elem + std::max<int64_t>(1, p.inv_tiles) * kSub * kSub * 8;
+  if (p.ginv_ok && opt_.solve_inverse) b += p.arena_elems * elem + vb(p.gtiles);  // G and its tiles
   if (opt_.precision == 1)  // full CSR of A (row pointers, columns, sources, values), at most 2 nnz
     b += (int64_t)(n_ + 1) * 8 + 2 * nnzA_ * (4 + 8 + 8);
   b += 2 * (int64_t)std::max(1, nrhs) * kRedBlocks * 8;
-  b += (int64_t)n_ * std::max(1, nrhs) * (2 * elem + (opt_.precision == 0 ? 3 : 6) * 8) +
+  b += (int64_t)n_ * std::max(1, nrhs) * (3 * elem + (opt_.precision == 0 ? 3 : 6) * 8) +
        48 * std::max(1, nrhs);
   return b;
 }
@@ -366,6 +433,7 @@ void Engine::ensure_device() {
   d_tasks_[TA_FUSED] = dupload(plan_.fused_tasks, s);
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
   d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
+  if (use_ginv()) d_tasks_[TA_GINV] = dupload(plan_.gtiles, s);
   d_info_ = dalloc<int>(1);
   if (!plan_.fused_tasks.empty()) {  // load counters of the fused block factor + TRSM
     d_fcnt_ = dalloc<int>((size_t)plan_.nsuper);
@@ -428,6 +496,13 @@ void Engine::ensure_device() {
     d_arena32_ = dalloc<float>(plan_.arena_elems);
   // inverted 16x16 diagonal tiles (double-sized so both precisions fit)
   d_dinv_ = dalloc<double>(std::max<int64_t>(1, plan_.inv_tiles) * kSub * kSub);
+  if (use_ginv()) {
+    if (opt_.precision == 0)
+      d_garena_ = dalloc<double>(plan_.arena_elems);
+    else
+      d_garena_ = dalloc<float>(plan_.arena_elems);
+  }
+  ginv_ready_ = false;
   HIP_OK(hipStreamSynchronize(s));
   device_ready_ = true;
 }
@@ -782,6 +857,7 @@ int Engine::factorize() {
   auto t0 = std::chrono::steady_clock::now();
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
+  ginv_ready_ = false;
   if (opt_.precision == 0) {
     run_graph(fgraph_, 1, [&] {
       assemble(d_arena64_);
@@ -825,6 +901,7 @@ void Engine::ensure_solve_capacity(int nrhs) {
   pending_graphs_.clear();
   dfree(d_Y_);
   dfree(d_X_);
+  dfree(d_Xf_);
   dfree(d_b_);
   dfree(d_x_);
   dfree(d_r_);
@@ -838,6 +915,7 @@ void Engine::ensure_solve_capacity(int nrhs) {
   size_t cnt = (size_t)n_ * nrhs;
   HIP_OK(hipMalloc(&d_Y_, std::max<size_t>(1, cnt) * tsz));
   HIP_OK(hipMalloc(&d_X_, std::max<size_t>(1, cnt) * tsz));
+  if (use_ginv()) HIP_OK(hipMalloc(&d_Xf_, std::max<size_t>(1, cnt) * tsz));
   d_b_ = dalloc<double>(cnt);
   d_x_ = dalloc<double>(cnt);
   d_r_ = dalloc<double>(cnt);
@@ -913,7 +991,89 @@ void Engine::solve_sequence_df(const T* arena, T* Y, T* X, int nrhs) {
 }
 
 template <typename T>
-void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs) {
+void Engine::ensure_ginv(const T* arena) {
+  if (!use_ginv() || ginv_ready_) return;
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const Task4* tiles = (const Task4*)d_tasks_[TA_GINV];
+  T* G = (T*)d_garena_;
+  const T* dinv = (const T*)d_dinv_;
+  for (const LaunchRange& r : plan_.gprep_seq) {
+    const unsigned g = (unsigned)r.count;
+    launch_begin(12, r.count, nullptr, -1);
+    switch (r.aux) {
+      case 1: dev::k_gprep<T, 1><<<g, 256, 0, stream_>>>(sn, tiles + r.first, arena, dinv, G); break;
+      case 2: dev::k_gprep<T, 2><<<g, 256, 0, stream_>>>(sn, tiles + r.first, arena, dinv, G); break;
+      case 4: dev::k_gprep<T, 4><<<g, 256, 0, stream_>>>(sn, tiles + r.first, arena, dinv, G); break;
+      case 6: dev::k_gprep<T, 6><<<g, 256, 0, stream_>>>(sn, tiles + r.first, arena, dinv, G); break;
+      case 10: dev::k_gprep<T, 10><<<g, 256, 0, stream_>>>(sn, tiles + r.first, arena, dinv, G); break;
+      default: throw std::logic_error("bad diagonal-block bucket");
+    }
+    launch_end(12);
+  }
+  HIP_OK(hipGetLastError());
+  ginv_ready_ = true;
+}
+
+// Partitioned-inverse sweeps: X zeroed, one forward launch per level
+// (Y accumulates the updates, Xf receives X_I), one backward launch per
+// level (partial sums into X).
+template <typename T>
+void Engine::ginv_sequence(T* Y, T* X, T* Xf, int nrhs) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const Task4* tiles = (const Task4*)d_tasks_[TA_GINV];
+  const T* G = (const T*)d_garena_;
+  const T* arena = opt_.precision == 0 ? (const T*)(void*)d_arena64_ : (const T*)(void*)d_arena32_;
+  const bool ldl = opt_.ldl != 0;
+  const int64_t cnt = (int64_t)n_ * nrhs;
+  dev::k_fill_zero<T><<<grid_for(cnt), 256, 0, stream_>>>(X, cnt);
+  const unsigned gy = (unsigned)((nrhs + 15) / 16);
+  for (const LaunchRange& r : plan_.gfwd_seq) {
+    const dim3 grid((unsigned)r.count, gy);
+    const Task4* t = tiles + r.first;
+    launch_begin(10, r.count, nullptr, r.level);
+#define SPK_GF(NBM)                                                                                        \
+  case NBM:                                                                                                \
+    if (ldl)                                                                                               \
+      dev::k_gsolve_fwd<T, true, NBM><<<grid, 256, 0, stream_>>>(sn, t, G, arena, d_lindx0_, Y, Xf, nrhs); \
+    else                                                                                                   \
+      dev::k_gsolve_fwd<T, false, NBM><<<grid, 256, 0, stream_>>>(sn, t, G, arena, d_lindx0_, Y, Xf, nrhs); \
+    break;
+    switch (r.aux) {
+      SPK_GF(1)
+      SPK_GF(2)
+      SPK_GF(4)
+      SPK_GF(6)
+      SPK_GF(10)
+      default:
+        throw std::logic_error("bad diagonal-block bucket");
+    }
+#undef SPK_GF
+    launch_end(10);
+  }
+  for (const LaunchRange& r : plan_.gbwd_seq) {
+    const dim3 grid((unsigned)r.count, gy);
+    const Task4* t = tiles + r.first;
+    launch_begin(11, r.count, nullptr, r.level);
+    switch (r.aux) {
+      case 1: dev::k_gsolve_bwd<T, 1><<<grid, 256, 0, stream_>>>(sn, t, G, d_lindx0_, Xf, X, nrhs); break;
+      case 2: dev::k_gsolve_bwd<T, 2><<<grid, 256, 0, stream_>>>(sn, t, G, d_lindx0_, Xf, X, nrhs); break;
+      case 4: dev::k_gsolve_bwd<T, 4><<<grid, 256, 0, stream_>>>(sn, t, G, d_lindx0_, Xf, X, nrhs); break;
+      case 6: dev::k_gsolve_bwd<T, 6><<<grid, 256, 0, stream_>>>(sn, t, G, d_lindx0_, Xf, X, nrhs); break;
+      case 10: dev::k_gsolve_bwd<T, 10><<<grid, 256, 0, stream_>>>(sn, t, G, d_lindx0_, Xf, X, nrhs); break;
+      default: throw std::logic_error("bad diagonal-block bucket");
+    }
+    launch_end(11);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+template <typename T>
+void Engine::solve_sequence(const T* arena, T* Y, T* X, int nrhs, T* Xf) {
+  if (use_ginv()) {
+    if (!ginv_ready_) throw std::logic_error("partitioned-inverse solve used before its preparation");
+    ginv_sequence<T>(Y, X, Xf ? Xf : (T*)d_Xf_, nrhs);
+    return;
+  }
   if (use_df()) {
     solve_sequence_df<T>(arena, Y, X, nrhs);
     return;
@@ -1048,6 +1208,149 @@ void Engine::pcg_restart(int nrhs) {
   HIP_OK(hipGetLastError());
 }
 
+// GMRES-based refinement for the FP32 static LDL^T factor of S A S
+// (indefinite A): restarted GMRES(m) in FP64 on A x = b with the right
+// preconditioner M^{-1} = S (L D L^T)^{-1} S, one Krylov space per
+// right-hand side, batched over the right-hand sides.  Each step: z_k =
+// M^{-1} v_k (stored), w = A z_k, classical Gram-Schmidt applied twice
+// against v_0..v_k, v_{k+1} = w / ||w||; the (k+2) x (k+1) Hessenberg columns
+// come to the host, where Givens rotations give the residual norm of every
+// right-hand side.  A cycle ends when every estimate is below half the
+// backward-error target (in the 2-norm, which bounds the infinity norm) or
+// after m steps; then x += Z y and the true residual decides.  Fixed-point
+// refinement x += M^{-1}(b - A x) needs the spectral radius of
+// I - M^{-1} A below one, which the FP32 static LDL^T of an indefinite
+// matrix does not guarantee; GMRES converges whenever the preconditioned
+// matrix's spectrum is clustered, also with a few eigenvalues away from one
+// (GMRES-IR, Carson and Higham 2017).
+void Engine::gmres_ir(int nrhs, const std::vector<double>& bnorm, SolveReport& rep) {
+  const int m = std::max(2, std::min(kGmresRestart, opt_.ir_max_iter));
+  const int64_t cnt = (int64_t)n_ * nrhs;
+  if (gm_cap_ < cnt * (2 * m + 1)) {
+    dfree(d_gmV_);
+    d_gmV_ = dalloc<double>((size_t)cnt * (2 * m + 1));  // V_0..V_m then Z_0..Z_{m-1}
+    gm_cap_ = cnt * (2 * m + 1);
+  }
+  // h: the two Gram-Schmidt coefficient sets and the norm of step k,
+  // (2 (k+1) + 1) x nrhs <= (2 m + 1) x nrhs; then the partial sums of up to
+  // m vectors
+  const int64_t hsize = (int64_t)(2 * m + 2) * nrhs;
+  const int64_t need = hsize + (int64_t)kRedBlocks * m * nrhs;
+  if (gm_hcap_ < need) {
+    dfree(d_gmh_);
+    gm_hcap_ = need;
+    d_gmh_ = dalloc<double>((size_t)gm_hcap_);
+  }
+  double* V = d_gmV_;
+  double* Z = d_gmV_ + cnt * (m + 1);
+  double* h = d_gmh_;
+  double* part = d_gmh_ + hsize;
+  double* w = d_q_;
+  const unsigned g = grid_for(cnt);
+  std::vector<double> red(2 * (size_t)nrhs);
+  auto berr_now = [&]() {
+    residual(d_b_, d_x_, d_r_, nrhs);
+    colmax2(d_r_, d_x_, nrhs, d_red_);
+    HIP_OK(hipMemcpyAsync(red.data(), d_red_, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    double be = 0.0;
+    for (int j = 0; j < nrhs; ++j) {
+      const double den = normA_inf_ * red[2 * j + 1] + bnorm[j];
+      be = std::max(be, den > 0 ? red[2 * j] / den : 0.0);
+    }
+    return be;
+  };
+  auto multidot = [&](int kv, const double* vec, double* out) {
+    k_multidot_part<<<dim3(kRedBlocks, (unsigned)nrhs, (unsigned)kv), 256, 0, stream_>>>(V, cnt, vec, n_, nrhs, part);
+    k_multidot_fin<<<(unsigned)(kv * nrhs), 256, 0, stream_>>>(part, out);
+  };
+  const int maxit = std::max(1, opt_.ir_max_iter);
+  int total = 0;
+  rep.converged = 0;
+  // x = 0, r = b on entry
+  std::vector<double> H((size_t)(m + 1) * m * nrhs), cs((size_t)m * nrhs), sn((size_t)m * nrhs),
+      gv((size_t)(m + 1) * nrhs), hk((size_t)(m + 2) * nrhs * 2), y((size_t)m * nrhs), tol(nrhs);
+  while (total < maxit) {
+    // v_0 = r / ||r||
+    coldot(d_r_, d_r_, nrhs, h);
+    k_scale_cols<<<g, 256, 0, stream_>>>(V, d_r_, h, n_, nrhs);
+    std::vector<double> beta2(nrhs);
+    HIP_OK(hipMemcpyAsync(beta2.data(), h, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (int j = 0; j < nrhs; ++j) {
+      gv[j] = std::sqrt(std::max(0.0, beta2[j]));
+      for (int i = 1; i <= m; ++i) gv[(size_t)i * nrhs + j] = 0.0;
+      tol[j] = 0.5 * opt_.ir_tol * (normA_inf_ * red[2 * j + 1] + bnorm[j]);
+    }
+    int kk = 0;
+    for (int k = 0; k < m && total < maxit; ++k) {
+      double* vk = V + (int64_t)k * cnt;
+      double* zk = Z + (int64_t)k * cnt;
+      precond(vk, zk, nrhs);
+      csr_mv(zk, nullptr, w, nrhs);
+      // classical Gram-Schmidt, twice; the two coefficient sets are added
+      multidot(k + 1, w, h);
+      k_gs_update<<<g, 256, 0, stream_>>>(w, V, cnt, h, k + 1, n_, nrhs);
+      multidot(k + 1, w, h + (int64_t)(k + 1) * nrhs);
+      k_gs_update<<<g, 256, 0, stream_>>>(w, V, cnt, h + (int64_t)(k + 1) * nrhs, k + 1, n_, nrhs);
+      double* nrm = h + (int64_t)2 * (k + 1) * nrhs;
+      coldot(w, w, nrhs, nrm);
+      k_scale_cols<<<g, 256, 0, stream_>>>(V + (int64_t)(k + 1) * cnt, w, nrm, n_, nrhs);
+      HIP_OK(hipGetLastError());
+      const size_t nh = (size_t)(2 * (k + 1) + 1) * nrhs;
+      HIP_OK(hipMemcpyAsync(hk.data(), h, nh * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      ++total;
+      kk = k + 1;
+      bool done = true;
+      for (int j = 0; j < nrhs; ++j) {
+        double* Hc = &H[((size_t)j * (m + 1)) * m];  // column-major (m+1) x m per right-hand side
+        auto Hat = [&](int r, int c) -> double& { return Hc[(size_t)c * (m + 1) + r]; };
+        for (int i = 0; i <= k; ++i) Hat(i, k) = hk[(size_t)i * nrhs + j] + hk[(size_t)(k + 1 + i) * nrhs + j];
+        Hat(k + 1, k) = std::sqrt(std::max(0.0, hk[(size_t)2 * (k + 1) * nrhs + j]));
+        for (int i = 0; i < k; ++i) {  // earlier rotations
+          const double c = cs[(size_t)i * nrhs + j], s = sn[(size_t)i * nrhs + j];
+          const double a = Hat(i, k), b = Hat(i + 1, k);
+          Hat(i, k) = c * a + s * b;
+          Hat(i + 1, k) = -s * a + c * b;
+        }
+        const double a = Hat(k, k), b = Hat(k + 1, k);
+        const double rr = std::hypot(a, b);
+        const double c = rr > 0 ? a / rr : 1.0, s = rr > 0 ? b / rr : 0.0;
+        cs[(size_t)k * nrhs + j] = c;
+        sn[(size_t)k * nrhs + j] = s;
+        Hat(k, k) = rr;
+        Hat(k + 1, k) = 0.0;
+        const double gk = gv[(size_t)k * nrhs + j];
+        gv[(size_t)k * nrhs + j] = c * gk;
+        gv[(size_t)(k + 1) * nrhs + j] = -s * gk;
+        if (std::fabs(gv[(size_t)(k + 1) * nrhs + j]) > tol[j]) done = false;
+      }
+      if (done) break;
+    }
+    // y = H^{-1} g per right-hand side (upper triangular kk x kk), x += Z y
+    for (int j = 0; j < nrhs; ++j) {
+      const double* Hc = &H[((size_t)j * (m + 1)) * m];
+      for (int i = kk - 1; i >= 0; --i) {
+        double t = gv[(size_t)i * nrhs + j];
+        for (int c = i + 1; c < kk; ++c) t -= Hc[(size_t)c * (m + 1) + i] * y[(size_t)c * nrhs + j];
+        const double d = Hc[(size_t)i * (m + 1) + i];
+        y[(size_t)i * nrhs + j] = d != 0.0 ? t / d : 0.0;
+      }
+    }
+    HIP_OK(hipMemcpyAsync(h, y.data(), (size_t)kk * nrhs * sizeof(double), hipMemcpyHostToDevice, stream_));
+    k_add_combo<<<g, 256, 0, stream_>>>(d_x_, Z, cnt, h, kk, n_, nrhs);
+    HIP_OK(hipGetLastError());
+    const double be = berr_now();  // also leaves r = b - A x for the next cycle
+    rep.iterations = total;
+    rep.backward_error = be;
+    if (be <= opt_.ir_tol) {
+      rep.converged = 1;
+      break;
+    }
+  }
+}
+
 SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   if (!factored_) throw std::logic_error("Solve called without a successful Factorize");
   if (opt_.solve_dataflow)
@@ -1056,6 +1359,10 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   HIP_OK(hipSetDevice(opt_.device));
   auto t0 = std::chrono::steady_clock::now();
   ensure_solve_capacity(nrhs);
+  if (opt_.precision == 0)
+    ensure_ginv<double>(d_arena64_);
+  else
+    ensure_ginv<float>(d_arena32_);
   int* df_abort = use_df() ? d_df_state_ + 2 + 4 * (size_t)plan_.nsuper : nullptr;
   if (df_abort) dev::k_fill_zero<int><<<1, 64, 0, stream_>>>(df_abort, 1);
   const size_t cnt = (size_t)n_ * nrhs;
@@ -1075,7 +1382,8 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
     // FP64 refinement around the FP32 factor of S A S (S = diag(s)).
     // SPD (LL^T): preconditioned conjugate gradients in FP64 with
     // M^{-1} = S (LL^T)^{-1} S, converging also where kappa(A) u_32 ~ 1;
-    // static LDL^T (indefinite): fixed-point refinement x += M^{-1}(b - A x).
+    // static LDL^T (indefinite): restarted GMRES in FP64 with the same
+    // preconditioner on the right (gmres_ir).
     // Both stop at the normwise backward error ir_tol, checked on the true
     // residual b - A x.
     std::vector<double> red(2 * (size_t)nrhs), bnorm(nrhs);
@@ -1117,19 +1425,7 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
         rep.backward_error = berr;
       }
     } else {
-      float* Y = (float*)d_Y_;
-      float* X = (float*)d_X_;
-      const int64_t kir = 4 * (int64_t)nrhs + 1;
-      for (int it = 0; it < maxit; ++it) {
-        run_graph(solve_graph(kir), kir, [&] { ir_step<float>(Y, X, nrhs); });
-        const double berr = read_berr();
-        rep.iterations = it + 1;
-        rep.backward_error = berr;
-        if (berr <= opt_.ir_tol) {
-          rep.converged = 1;
-          break;
-        }
-      }
+      gmres_ir(nrhs, bnorm, rep);
     }
   }
   int aborted = 0;

@@ -14,6 +14,7 @@
 namespace spk {
 
 constexpr int kRedBlocks = 64;  // blocks per column of the refinement reductions
+constexpr int kGmresRestart = 20;  // GMRES restart length of the LDL^T refinement
 
 struct EngineOptions {
   int ordering = ORD_MMD;
@@ -27,6 +28,7 @@ struct EngineOptions {
   int verbose = 0;
   int merge_updates = 1;  // 0 off, 1 on levels that fill the GPU, 2 every qualifying chain child
   int solve_dataflow = 0; // dependency-driven solve sweeps: disabled, requesting it is an error
+  int solve_inverse = 1;  // partitioned-inverse solve where every supernode is <= kStep wide
   int refine = 0;         // 10*ordflag + altflag of the supernode reordering, 0 = none
   std::vector<Int> user_perm;
 };
@@ -106,7 +108,22 @@ class Engine {
   template <typename T>
   void launch_diagtrsm(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
   template <typename T>
-  void solve_sequence(const T* arena, T* Y, T* X, int nrhs);
+  void solve_sequence(const T* arena, T* Y, T* X, int nrhs, T* Xf = nullptr);
+  template <typename T>
+  void ginv_sequence(T* Y, T* X, T* Xf, int nrhs);
+  // the partitioned-inverse solve applies to this plan and is switched on.
+  // Not for FP64 static LDL^T: explicit inverses of its unit-lower diagonal
+  // blocks (indefinite, static pivots) raise the backward error several
+  // times over block substitution (80^3-like shifted Laplacians: 6x on the
+  // host in exact block inverses, 26x measured at 48^3), beyond the
+  // drop-in's 10x bar; as the FP32 preconditioner under refinement it is
+  // used, the refinement restores the accuracy
+  bool use_ginv() const {
+    return plan_.ginv_ok && opt_.solve_inverse != 0 && (opt_.ldl == 0 || opt_.precision == 1);
+  }
+  // prepares G from the current factor once per factorization
+  template <typename T>
+  void ensure_ginv(const T* arena);
   template <typename T>
   void solve_sequence_df(const T* arena, T* Y, T* X, int nrhs);
   // The dependency-driven sweeps did not finish on MI355X (waits on counters
@@ -121,6 +138,7 @@ class Engine {
   void precond(const double* r, double* z, int nrhs);
   void pcg_iteration(int nrhs);
   void pcg_restart(int nrhs);
+  void gmres_ir(int nrhs, const std::vector<double>& bnorm, SolveReport& rep);
   void launch_begin(int kind, int64_t grid = 0, hipStream_t st = nullptr, int level = -1);
   void launch_end(int kind, hipStream_t st = nullptr);
 
@@ -172,7 +190,7 @@ class Engine {
   int32_t* d_mmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
   int32_t* d_invp0_ = nullptr;  // original -> permuted row, 0-based
-  void* d_tasks_[8] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused
+  void* d_tasks_[9] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused, partitioned-inverse tiles
   int* d_info_ = nullptr;
   int* d_fcnt_ = nullptr;  // per supernode: loads of the fused block factor + TRSM
   // dependency-driven solve
@@ -193,11 +211,14 @@ class Engine {
   double* d_scale_ = nullptr;  // equilibration s_i (FP32 mode)
   double* d_arena64_ = nullptr;
   void* d_dinv_ = nullptr;     // inverted 16x16 diagonal tiles of the factor
+  void* d_garena_ = nullptr;   // partitioned-inverse solve factor G (panel shapes, factor precision)
+  bool ginv_ready_ = false;    // G holds the current factor's inverse
   float* d_arena32_ = nullptr;
   // solve work
   int cap_nrhs_ = 0;
   void* d_Y_ = nullptr;
   void* d_X_ = nullptr;
+  void* d_Xf_ = nullptr;  // forward-sweep results of the partitioned-inverse solve
   double* d_b_ = nullptr;
   double* d_x_ = nullptr;
   double* d_r_ = nullptr;
@@ -215,6 +236,9 @@ class Engine {
   double* d_p_ = nullptr;
   double* d_q_ = nullptr;
   double* d_s_ = nullptr;  // per-RHS CG scalars
+  double* d_gmV_ = nullptr;  // GMRES basis V_0..V_m and preconditioned vectors Z_0..Z_{m-1}
+  double* d_gmh_ = nullptr;  // GMRES dot products, norms and partial sums
+  int64_t gm_cap_ = 0, gm_hcap_ = 0;
  public:
   double fp32_shift_ = 0.0;  // diagonal shift of S A S used by the last FP32 factor
  private:

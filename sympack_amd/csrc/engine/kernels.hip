@@ -1421,6 +1421,269 @@ __global__ __launch_bounds__(NT) void k_soffm_bwd(const SnInfo* __restrict__ sn,
 }
 
 // ---------------------------------------------------------------------------
+// Partitioned-inverse solve (supernodes at most kStep wide).  Per supernode
+// I (panel r x w, column-major, ld r) the solve factor
+//   G_I = [I_w; L_off] L_II^{-1}
+// (LDL: L_II unit lower) is kept in an arena of the panel's shape.  Then
+//   forward  [X_I; dY] = G_I Y_I,  Y[struct rows] -= dY   (LDL: X_I /= D)
+//   backward X_I = G_I^T [X_I; -X[struct rows]]
+// (X_I = L_II^{-T}(X_I - L_off^T X_R) = L_II^{-T} X_I - (L_off L_II^{-1})^T X_R),
+// one launch per etree level and sweep, no triangular recurrence on the
+// solve's critical path.  Explicit inverses of the <= 160-wide diagonal
+// blocks, as in the partitioned-inverse / selective-inversion methods
+// (Alvarado, Pothen, Schreiber; Raghavan); the reference substitutes
+// instead (forward_update / back_update, symPACKMatrix2D.hpp:1486-1797).
+// ---------------------------------------------------------------------------
+
+// Stage the strict lower tiles of the diagonal block (I > J) and the
+// inverse diagonal tiles (I == J) of supernode s in LDS (all 256 threads).
+template <typename T, int NBM>
+__device__ __forceinline__ void stage_block_and_inverses(const SnInfo& s, int64_t invb,
+                                                         const T* __restrict__ arena,
+                                                         const T* __restrict__ dinv, T* __restrict__ tl) {
+  constexpr int NTL = NBM * (NBM + 1) / 2;
+  const int w = s.w;
+  const int nb = (w + SUB - 1) / SUB;
+  const int64_t ld = s.r;
+  const T* __restrict__ P = arena + s.off;
+  const int tid = threadIdx.x;
+  const int r = tid & 15, c = tid >> 4;
+  const T* Dv = dinv + invb * (SUB * SUB);
+  T v[NTL];
+#pragma unroll
+  for (int I = 0; I < NBM; ++I)
+#pragma unroll
+    for (int J = 0; J <= I; ++J) {
+      if (I == J) {
+        v[tslot(I, J)] = Dv[(int64_t)min(I, nb - 1) * (SUB * SUB) + c * SUB + r];
+      } else {
+        const int row = min(SUB * I + r, w - 1), col = min(SUB * J + c, w - 1);
+        v[tslot(I, J)] = P[(int64_t)col * ld + row];
+      }
+    }
+#pragma unroll
+  for (int q = 0; q < NTL; ++q) keep(v[q]);
+#pragma unroll
+  for (int I = 0; I < NBM; ++I)
+#pragma unroll
+    for (int J = 0; J <= I; ++J) {
+      const int row = SUB * I + r, col = SUB * J + c;
+      const bool in = (I == J) ? (I < nb) : (row < w && col < w);
+      tl[tslot(I, J) * TSZ + c * TSTR + r] = in ? v[tslot(I, J)] : T(0);
+    }
+}
+
+// G rows [64 rt, 64 rt + 64) of supernode t.a: X L_II = B with B = e_a^T for
+// the diagonal rows a < w and B = L(a, :) below, i.e. the block backward
+// substitution Y_K = inv(L_KK)^T (B_K^T - sum_{J>K} L_JK^T Y_J) on Y = X^T,
+// 16 rows per wave (Y^T in MFMA accumulators, as in the panel TRSM).  Task
+// {sn, row tile, inverse-tile base, -}.
+template <typename T, int NBM>
+__global__ __launch_bounds__(NT) void k_gprep(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks,
+                                              const T* __restrict__ arena, const T* __restrict__ dinv,
+                                              T* __restrict__ garena) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NTL = NBM * (NBM + 1) / 2;
+  __shared__ __attribute__((aligned(16))) T tl[NTL * TSZ];
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  stage_block_and_inverses<T, NBM>(s, t.c, arena, dinv, tl);
+  __syncthreads();
+  const int w = s.w;
+  const int nb = (w + SUB - 1) / SUB;
+  const int64_t ld = s.r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int R0 = t.b * TILE + 16 * wave;
+  if (R0 >= s.r) return;  // no barrier follows
+  const int prow = min(R0 + (lane & 15), s.r - 1);
+  const bool prow_ok = R0 + (lane & 15) < s.r;
+  const T* __restrict__ P = arena + s.off;
+  v4 y[NBM];
+#pragma unroll
+  for (int K = 0; K < NBM; ++K)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int col = SUB * K + Mfma<T>::row(lane, v);
+      const T lv = P[(int64_t)min(col, w - 1) * ld + prow];
+      y[K][v] = col >= w ? T(0) : (prow < w ? T(prow == col ? 1 : 0) : lv);
+    }
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) asm volatile("" : "+v"(y[K]));
+#pragma unroll
+  for (int Ks = 0; Ks < NBM; ++Ks) {
+    const int K = NBM - 1 - Ks;
+    if (K < nb) {
+      v4 y2 = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int J = 0; J < NBM; ++J) {
+        if (J > K && J < nb) {
+          const T* L = tl + tslot(J, K) * TSZ + (lane & 15) * TSTR;  // L_JK^T[m][k] = L_JK[k][m]
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int kp = Mfma<T>::row(lane, kk);
+            if (J & 1)
+              y2 = Mfma<T>::mma(-L[kp], y[J][kk], y2);
+            else
+              y[K] = Mfma<T>::mma(-L[kp], y[J][kk], y[K]);
+          }
+        }
+      }
+      y[K] += y2;
+      const T* Li = tl + tslot(K, K) * TSZ + (lane & 15) * TSTR;  // inv(L_KK)^T
+      v4 z = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int kp = Mfma<T>::row(lane, kk);
+        z = Mfma<T>::mma(Li[kp], y[K][kk], z);
+      }
+      y[K] = z;
+    }
+  }
+  if (!prow_ok) return;
+  T* __restrict__ G = garena + s.off + prow;
+#pragma unroll
+  for (int K = 0; K < NBM; ++K) {
+    if (K < nb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = SUB * K + Mfma<T>::row(lane, v);
+        if (col < w) G[(int64_t)col * ld] = y[K][v];
+      }
+    }
+  }
+}
+
+// Forward level: rows [64 rt, 64 rt + 64) of [X_I; dY] = G_I Y_I for 16
+// right-hand sides (blockIdx.y), wave q owning 16 rows.  Every G and Y_I
+// operand is loaded before the first product (one memory round trip).
+// Rows a < w go to Xf (LDL: divided by D_a), rows below are subtracted from
+// Y at their structure rows with atomics.  Y_I of a level's supernodes is
+// never the target of the same level's updates (targets are ancestors).
+template <typename T, bool LDL, int NBM>
+__global__ __launch_bounds__(NT) void k_gsolve_fwd(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ garena, const T* __restrict__ arena,
+                                                   const int32_t* __restrict__ lindx0, T* Y, T* __restrict__ Xf,
+                                                   int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NQ = 4 * NBM;  // k chunks of 4
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int a0 = t.b * TILE + 16 * wave;
+  if (a0 >= s.r) return;
+  const int w = s.w;
+  const int64_t ld = s.r;
+  const int mi = lane & 15, g = lane >> 4;
+  const int ac = min(a0 + mi, s.r - 1);
+  const int j0 = blockIdx.y * SUB;
+  const int jn = j0 + mi;
+  const int jc = min(jn, nrhs - 1);
+  const T jmask = T(jn < nrhs);
+  const T* __restrict__ Ga = garena + s.off + ac;           // G(a, k) at Ga[k ld]
+  const T* __restrict__ Yk = Y + (int64_t)s.fc * nrhs + jc;  // Y_I(k, j) at Yk[k nrhs]
+  T av[NQ], bv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int k = 4 * q + g;
+    const int kc = min(k, w - 1);
+    av[q] = Ga[(int64_t)kc * ld];
+    bv[q] = Yk[(int64_t)kc * nrhs];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    keep(av[q]);
+    keep(bv[q]);
+  }
+  v4 acc = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (4 * q < w) {  // uniform: chunks wholly beyond w are skipped
+      const T km = T(4 * q + g < w);
+      acc = Mfma<T>::mma(av[q] * km, bv[q] * jmask, acc);
+    }
+  }
+  if (jn >= nrhs) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int a = a0 + Mfma<T>::row(lane, v);
+    if (a < w) {
+      T x = acc[v];
+      if (LDL) x /= arena[s.off + (int64_t)a * ld + a];
+      Xf[(int64_t)(s.fc + a) * nrhs + jn] = x;
+    } else if (a < s.r) {
+      atomic_add(Y + (int64_t)lindx0[s.lx + a] * nrhs + jn, -acc[v]);
+    }
+  }
+}
+
+// Backward level: X_I += G_I(tile rows, :)^T V(tile rows) for 16 right-hand
+// sides, V(a) = Xf(a) on the diagonal rows and -X(struct row a) below (the
+// ancestors' final values).  Lane (m, g) reads rows 16g..16g+15 of one G
+// column (a cache line), wave q the 16-column blocks q, q+4, q+8; every load
+// is issued before the first product; partial sums of the tile go to X_I
+// with atomics (X zeroed before the sweep).
+template <typename T, int NBM>
+__global__ __launch_bounds__(NT) void k_gsolve_bwd(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks,
+                                                   const T* __restrict__ garena,
+                                                   const int32_t* __restrict__ lindx0, const T* __restrict__ Xf,
+                                                   T* X, int nrhs) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int MKB = (NBM + 3) / 4;  // column blocks per wave
+  const Task4 t = tasks[blockIdx.x];
+  const SnInfo s = sn[t.a];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int w = s.w;
+  const int nkb = (w + 15) >> 4;
+  if (wave >= nkb) return;
+  const int a0 = t.b * TILE;
+  const int mi = lane & 15, g = lane >> 4;
+  const int j0 = blockIdx.y * SUB;
+  const int jn = j0 + mi;
+  const int jc = min(jn, nrhs - 1);
+  const T jmask = T(jn < nrhs);
+  const int64_t ld = s.r;
+  int rowi[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) rowi[st] = lindx0[s.lx + min(a0 + 16 * g + st, s.r - 1)];
+  T av[MKB][16];
+#pragma unroll
+  for (int i = 0; i < MKB; ++i) {
+    const int kcol = min((wave + 4 * i) * 16 + mi, w - 1);
+    const T* __restrict__ Gc = garena + s.off + (int64_t)kcol * ld;
+#pragma unroll
+    for (int st = 0; st < 16; ++st) av[i][st] = Gc[min(a0 + 16 * g + st, s.r - 1)];
+  }
+  T bv[16];
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const int a = a0 + 16 * g + st;
+    const T* src = a < w ? Xf : X;
+    const T sg = a < w ? T(1) : T(-1);
+    bv[st] = src[(int64_t)rowi[st] * nrhs + jc] * (jmask * T(a < s.r) * sg);
+  }
+#pragma unroll
+  for (int i = 0; i < MKB; ++i)
+#pragma unroll
+    for (int st = 0; st < 16; ++st) keep(av[i][st]);
+#pragma unroll
+  for (int i = 0; i < MKB; ++i) {
+    const int kb = wave + 4 * i;
+    if (kb < nkb) {
+      v4 acc = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[i][st], bv[st], acc);
+      if (jn < nrhs) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int k = kb * 16 + Mfma<T>::row(lane, v);
+          if (k < w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, acc[v]);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Dependency-driven solve (one persistent launch per sweep, no level
 // barriers).  Workgroups take tasks in a topological order from a global
 // ticket counter; a task waits only for the tasks it depends on, which hold

@@ -378,6 +378,44 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
     mcount[L] = (int64_t)p.soffm_tasks.size() - mfirst[L];
   }
+  // partitioned-inverse solve: 64-row tiles over all rows of every panel,
+  // grouped by level (solve sweeps) and by diagonal-block bucket (the
+  // preparation, one launch per bucket over all supernodes)
+  p.ginv_ok = p.max_w <= kStep;
+  if (p.ginv_ok) {
+    std::vector<int> lbucket(p.nlevels, 1);
+    for (int L = 0; L < p.nlevels; ++L)
+      for (Int I : by_level[L]) lbucket[L] = std::max(lbucket[L], diag_bucket((int)ceil_div(p.sn[I].w, kSub)));
+    std::vector<int64_t> gfirst(p.nlevels), gcount(p.nlevels);
+    for (int L = 0; L < p.nlevels; ++L) {
+      gfirst[L] = (int64_t)p.gtiles.size();
+      for (Int I : by_level[L])
+        for (int64_t rt = 0; rt < ceil_div(p.sn[I].r, kTile); ++rt)
+          p.gtiles.push_back({I, (int32_t)rt, (int32_t)p.inv_base[I], 0});
+      gcount[L] = (int64_t)p.gtiles.size() - gfirst[L];
+    }
+    for (int L = 0; L < p.nlevels; ++L) {
+      LaunchRange f{10, gfirst[L], gcount[L], lbucket[L], L};
+      if (gcount[L]) p.gfwd_seq.push_back(f);
+    }
+    for (int L = p.nlevels - 1; L >= 0; --L) {
+      LaunchRange b{11, gfirst[L], gcount[L], lbucket[L], L};
+      if (gcount[L]) p.gbwd_seq.push_back(b);
+    }
+    // preparation: the same tiles, regrouped by the bucket of their own
+    // supernode (appended after the level-ordered tiles)
+    for (int bk : {1, 2, 4, 6, 10}) {
+      LaunchRange r{12, (int64_t)p.gtiles.size(), 0, bk, -1};
+      for (Int I = 0; I < ns; ++I) {
+        if (diag_bucket((int)ceil_div(p.sn[I].w, kSub)) != bk) continue;
+        for (int64_t rt = 0; rt < ceil_div(p.sn[I].r, kTile); ++rt)
+          p.gtiles.push_back({I, (int32_t)rt, (int32_t)p.inv_base[I], 0});
+      }
+      r.count = (int64_t)p.gtiles.size() - r.first;
+      if (r.count) p.gprep_seq.push_back(r);
+    }
+  }
+
   // dependency-driven solve task lists
   p.df_ok = p.max_w <= kStep;
   if (p.df_ok) {

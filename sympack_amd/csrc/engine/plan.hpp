@@ -77,7 +77,8 @@ struct RelGroup {          // mirrored on the device
 struct LaunchRange {
   int kind;          // 0 diagonal factor, 7 panel TRSM, 9 both fused, 1 in-panel update,
                      // 2 Schur update,
-                     // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag
+                     // 3 forward diag, 4 forward off-diag, 5 backward off-diag, 6 backward diag,
+                     // 10 forward / 11 backward partitioned-inverse solve, 12 its preparation
   int64_t first;     // first task in that kind's task array
   int64_t count;     // number of tasks (= grid size)
   int aux = 0;       // kinds 0, 7, 9, 3, 6: diagonal-block bucket (diag_bucket);
@@ -121,6 +122,16 @@ struct Plan {
   std::vector<Task4> sdiag_tasks;  // {sn, 0, 0, 0}: diagonal-block solves
   std::vector<Task4> soffm_tasks;  // {sn, row_tile, 0, 0}: 64-row tiles of the off-diagonal panel
   std::vector<LaunchRange> fwd_seq, bwd_seq;  // kind 3/6 aux: diag_bucket of the level, 0 = w > kStep
+  // Partitioned-inverse solve, available when every supernode is at most
+  // kStep wide: G_I = [I; L_off] L_II^{-1} (r_I x w_I, the panel's shape and
+  // offsets, in its own arena) is prepared once per factorization; then each
+  // etree level costs one forward launch [X_I; dY] = G_I Y_I and one
+  // backward launch X_I = G_I^T [X_I; -X_rows].  Tasks {sn, 64-row tile,
+  // inverse-tile base, 0} over all rows of every panel; kinds 10/11 per
+  // level, kind 12 (preparation) per diagonal-block bucket, aux = bucket.
+  bool ginv_ok = false;
+  std::vector<Task4> gtiles;
+  std::vector<LaunchRange> gfwd_seq, gbwd_seq, gprep_seq;
   // dependency-driven solve (one persistent launch per sweep), available when
   // every supernode fits one diagonal block (w <= kStep).  Tasks
   // {kind, sn, row tile, first target}: forward D(s) = 0 / O(s,rt) = 1 in

@@ -51,7 +51,7 @@ class symPACKOptions:
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
                  profile=False, device=0, merge_updates=1, solve_dataflow=False,
-                 order_refinement_str="NO"):
+                 order_refinement_str="NO", solve_inverse=True):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -67,6 +67,8 @@ class symPACKOptions:
         # supernode reordering (symPACKMatrix2D.hpp:3090-3111): NO, SET,
         # SET10, SET11, SET20, SET21, SET30, SET31
         self.order_refinement_str = order_refinement_str
+        # partitioned-inverse solve where every supernode is <= 160 wide
+        self.solve_inverse = solve_inverse
 
     @classmethod
     def from_args(cls, args):
@@ -83,7 +85,8 @@ class symPACKOptions:
                    relax=r, precision="FP32_IR" if o.precision else "FP64",
                    ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
                    merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow),
-                   order_refinement_str="SET%d" % o.refine if o.refine else "NO")
+                   order_refinement_str="SET%d" % o.refine if o.refine else "NO",
+                   solve_inverse=bool(o.solve_inverse))
 
     def to_c(self):
         lib = _lib.load()
@@ -106,6 +109,7 @@ class symPACKOptions:
         o.device = self.device
         o.merge_updates = int(self.merge_updates)
         o.solve_dataflow = 1 if self.solve_dataflow else 0
+        o.solve_inverse = 1 if self.solve_inverse else 0
         r = self.order_refinement_str
         if r.startswith("TSPB"):
             raise ValueError("-refine TSPB is not supported; use SET, SET10 .. SET31")
@@ -140,6 +144,32 @@ class DistSparseMatrix:
                                         _ptr(rowind, ctypes.c_int32), _ptr(nzval, ctypes.c_double)),
                    "ReadMatrix")
         return cls(n.value, colptr, rowind, nzval, baseval=1)
+
+    @classmethod
+    def read(cls, filename, informat="HARWELL_BOEING"):
+        """ReadMatrix (utility.hpp:1693-1716): informat "CSC" (binary),
+        "HB" / "RB" / "HARWELL_BOEING" or "matrix" (text); other names raise
+        ValueError("Unknown matrix format.")."""
+        lib = _lib.load()
+        n = ctypes.c_int32()
+        nnz = ctypes.c_int64()
+        f, fmt = filename.encode(), informat.encode()
+        _lib.check(lib.spk_read_matrix(f, fmt, ctypes.byref(n), ctypes.byref(nnz)), "ReadMatrix")
+        colptr = np.zeros(n.value + 1, np.int64)
+        rowind = np.zeros(nnz.value, np.int32)
+        nzval = np.zeros(nnz.value, np.float64)
+        _lib.check(lib.spk_read_matrix_data(f, fmt, _ptr(colptr, ctypes.c_int64), _ptr(rowind, ctypes.c_int32),
+                                            _ptr(nzval, ctypes.c_double)), "ReadMatrix")
+        return cls(n.value, colptr, rowind, nzval, baseval=1)
+
+    def write(self, filename, outformat):
+        """Write the lower triangle as "CSC", "HB"/"RB"/"HARWELL_BOEING" or "matrix"."""
+        lib = _lib.load()
+        cp = np.ascontiguousarray(self.colptr, np.int64)
+        ri = np.ascontiguousarray(self.rowind, np.int32)
+        _lib.check(lib.spk_write_matrix(filename.encode(), outformat.encode(), self.size, _ptr(cp, ctypes.c_int64),
+                                        _ptr(ri, ctypes.c_int32),
+                                        _ptr(np.ascontiguousarray(self.nzval), ctypes.c_double)), "write")
 
     def write_rb(self, filename, title="sympack_amd"):
         lib = _lib.load()

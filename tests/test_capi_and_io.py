@@ -132,3 +132,55 @@ def test_cpp_adapter_symbolic_and_errors():
     out = subprocess.run([exe, "symbolic"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "adapter_c1: OK" in out.stdout
+
+
+@pytest.mark.parametrize("fmt", ["CSC", "matrix", "RB", "HB", "HARWELL_BOEING"])
+def test_matrix_formats_round_trip(tmp_path, fmt):
+    n, cp, ri, v = M.diffusion_3d_7pt(6, seed=4)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    f = str(tmp_path / ("a." + fmt))
+    A.write(f, fmt)
+    B = sp.DistSparseMatrix.read(f, fmt)
+    assert B.size == n and B.baseval == 1
+    assert np.array_equal(B.colptr - 1, cp)
+    assert np.array_equal(B.rowind - 1, ri)
+    assert np.array_equal(B.nzval, v)
+
+
+def test_csc_binary_layout_and_row_sorting(tmp_path):
+    """The byte layout ParaReadDistSparseMatrix's offsets read
+    (utility.hpp:1087-1190): int32 n, nnz, then count + colptr, count + rowind,
+    count + nzval; rows come back sorted within each column, values following."""
+    f = tmp_path / "m.csc"
+    n, nnz = 3, 5
+    colptr = np.array([1, 3, 5, 6], np.int32)
+    rowind = np.array([3, 1, 3, 2, 3], np.int32)  # column 1 unsorted
+    nzval = np.array([-1.0, 4.0, -2.0, 5.0, 6.0])
+    with open(f, "wb") as fh:
+        np.array([n, nnz, n + 1], np.int32).tofile(fh)
+        colptr.tofile(fh)
+        np.array([nnz], np.int32).tofile(fh)
+        rowind.tofile(fh)
+        np.array([nnz], np.int32).tofile(fh)
+        nzval.tofile(fh)
+    A = sp.DistSparseMatrix.read(str(f), "CSC")
+    assert list(A.colptr) == [1, 3, 5, 6]
+    assert list(A.rowind) == [1, 3, 2, 3, 3]
+    assert list(A.nzval) == [4.0, -1.0, 5.0, -2.0, 6.0]
+
+
+def test_matrix_text_full_symmetric_is_reduced_to_lower(tmp_path):
+    """A "matrix" file with entries above the diagonal is the full symmetric
+    matrix: the lower triangle is kept (ToLowerTriangular, impl:310-340)."""
+    f = tmp_path / "full.matrix"
+    # [[4,-1,0],[-1,4,-2],[0,-2,6]] stored in full
+    f.write_text("3 3\n7 0\n1 3 6 8\n1 2\n1 2 3\n2 3\n4 -1\n-1 4 -2\n-2 6\n")
+    A = sp.DistSparseMatrix.read(str(f), "matrix")
+    assert list(A.colptr) == [1, 3, 5, 6]
+    assert list(A.rowind) == [1, 2, 2, 3, 3]
+    assert list(A.nzval) == [4.0, -1.0, 4.0, -2.0, 6.0]
+
+
+def test_unknown_matrix_format():
+    with pytest.raises(ValueError, match="Unknown matrix format"):
+        sp.DistSparseMatrix.read("/nonexistent", "MATRIX_MARKET")

@@ -132,3 +132,34 @@ def test_serving_stream_shapes_alone_equal_mix(prec):
         else:
             assert be_a <= 1e-12 and be_b <= 1e-12
             assert rel <= 1e-8
+
+
+def test_ldl_80cube_fp32_gmres_ir_converges():
+    """Static LDL^T of the shifted 7-point Laplacian L_h - 0.3 I on 80^3
+    (indefinite), FP32 factor + FP64 GMRES refinement, 64 right-hand sides:
+    backward error <= 1e-12 (computed here on the host) and the refinement
+    reports convergence."""
+    n, cp, ri, v = M.shifted_laplacian_3d_7pt(80, 0.3)
+    A = sp.DistSparseMatrix(n, cp, ri, v)
+    rp = sp.RelaxationParameters()
+    rp.SetMaxSize(150)
+    Mx = sp.symPACKMatrix2D()
+    Mx.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", relax=rp, precision="FP32_IR",
+                              ir_tol=1e-12))
+    Mx.SymbolicFactorization(A)
+    Mx.DistributeMatrix(A)
+    print("80^3 shifted Laplacian LDL^T FP32 + GMRES, nrhs=64", flush=True)
+    B = np.random.default_rng(2).uniform(-1, 1, (n, 64))
+    for rep in range(2):
+        Mx.Factorize()
+        Mx.Solve(B, 64)
+        X = Mx.GetSolution(64).reshape(n, 64)
+        st = Mx.stats()
+        F = A.to_scipy_full()
+        R = B - F @ X
+        normA = abs(F).sum(axis=1).max()
+        be = float(np.max(np.abs(R).max(axis=0) / (normA * np.abs(X).max(axis=0) + np.abs(B).max(axis=0))))
+        print("factorization %d: steps %d converged %d berr %.2e (library %.2e)" %
+              (rep, st["ir_iterations"], st["ir_converged"], be, st["backward_error"]), flush=True)
+        assert st["ir_converged"] == 1
+        assert be <= 1e-12

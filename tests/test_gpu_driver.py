@@ -32,6 +32,19 @@ def test_driver_laplacian_64(tmp_path, ordering):
     assert flops > 0
 
 
+@pytest.mark.parametrize("fmt", ["CSC", "matrix"])
+def test_driver_reads_csc_and_matrix_formats(tmp_path, fmt):
+    n, cp, ri, v = M.laplacian_2d_5pt(64)
+    f = str(tmp_path / ("lap64." + fmt))
+    sp.DistSparseMatrix(n, cp, ri, v).write(f, fmt)
+    out = subprocess.run([DRIVER, "-in", f, "-inf", fmt, "-ordering", "MMD", "-nrhs", "1"],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    res = float(re.search(r"Norm of residual after factorization/solve is (\S+)", out.stdout).group(1))
+    assert res < 1e-12
+
+
 def test_driver_rejects_unknown_format(tmp_path):
     out = subprocess.run([DRIVER, "-in", str(tmp_path / "x"), "-inf", "MATRIX_MARKET"], capture_output=True,
                          text=True, timeout=60)
