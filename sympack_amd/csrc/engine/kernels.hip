@@ -766,7 +766,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     alim = s.r;
     blim = s.r;
   }
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wb = wave >> 1, wa = wave & 1;
   v4 acc[2][2];
 #pragma unroll
@@ -847,6 +847,10 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     }
     mk = m;
   };
+  // On a diagonal tile (ti == tj) the wave holding rows b of the upper half
+  // and columns a of the lower half (wb > wa) only produces entries a < b,
+  // which are never stored: it skips its products.
+  const bool idle = (ti == tj) && (wb > wa);
   fetch(q_lo);
   for (int qc = q_lo; qc < q_hi; ++qc) {
 #pragma unroll
@@ -856,9 +860,14 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
     }
     if (LDL && tid < KC) Dk[tid] = ((mk >> (2 * NQ)) & 1u) ? dv : T(0);
     __syncthreads();
+    // k steps of 4 that hold valid columns in this chunk (the last chunk of
+    // each part is partial: its zero-filled tail is not multiplied)
+    const int krem = qc < nq1 ? K - qc * KC : K2 - (qc - nq1) * KC;
+    const int nkk = idle ? 0 : min(KC / 4, (krem + 3) >> 2);
     if (qc + 1 < q_hi) fetch(qc + 1);  // next chunk in flight during the MFMAs
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
+      if (kk >= nkk) break;  // uniform over the wave
       const int k = kk * 4 + (lane >> 4);
       T bf[2], af[2];
 #pragma unroll
