@@ -166,6 +166,16 @@ int spk_factorize(spk_handle h);
  * not modified; rhs_size is unused, as in the reference's 2D path. */
 int spk_solve(spk_handle h, const double* rhs, int nrhs, int64_t rhs_size);
 int spk_solve_device(spk_handle h, const double* d_rhs, int nrhs);
+/* Allocation-free solve: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops): Solve + GetSolution of a factored FP64 handle from device-resident
+ * d_rhs into device-resident d_x (both column-major n x nrhs), with the work
+ * vectors in a static per-device array of 128 MiB instead of hipMalloc'd
+ * buffers, kernels launched directly (no graph instantiation).  Limit:
+ * 3 n nrhs doubles <= 128 MiB (n nrhs <= 5592405); larger requests, an
+ * FP32_IR handle or profile mode return -2.  Calls that overlap on one GPU
+ * share the array and race. */
+int spk_solve_device_noalloc(spk_handle h, const double* d_rhs, double* d_x, int nrhs);
 /* symPACKMatrix2D::GetSolution (symPACKMatrix2D.hpp:6713-6757). */
 int spk_get_solution(spk_handle h, double* B, int nrhs);
 int spk_get_solution_device(spk_handle h, double* d_B, int nrhs);

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("spk_solve_device", _I, [_P, _P, _I]),
     ("spk_get_solution", _I, [_P, _PD, _I]),
     ("spk_get_solution_device", _I, [_P, _P, _I]),
+    ("spk_solve_device_noalloc", _I, [_P, _P, _P, _I]),
     ("spk_get_stats", _I, [_P, ctypes.POINTER(SpkStats)]),
     ("spk_get_kernel_stats", _I, [_P, _I, _PI64, _PD]),
     ("spk_reset_kernel_stats", _I, [_P]),

@@ -311,6 +311,15 @@ int spk_solve_device(spk_handle h, const double* d_rhs, int nrhs) {
   });
 }
 
+int spk_solve_device_noalloc(spk_handle h, const double* d_rhs, double* d_x, int nrhs) {
+  if (!h || !d_rhs || !d_x) return fail("null argument");
+  if (nrhs <= 0) return fail("nrhs must be positive");
+  return guarded([&]() {
+    h->engine.solve_noalloc(d_rhs, d_x, nrhs);
+    return 0;
+  });
+}
+
 int spk_get_solution(spk_handle h, double* B, int nrhs) {
   if (!h || !B) return fail("null argument");
   return guarded([&]() {

@@ -1439,6 +1439,67 @@ SolveReport Engine::solve(const double* rhs, int nrhs, bool on_device) {
   return rep;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free FP64 solve.  The work vectors Y, X and Xf (n x nrhs each,
+// row-major, permuted numbering) live in one file-scope device array of
+// kNoAllocScratchBytes instead of buffers from hipMalloc, for callers that
+// cannot allocate (code under graph capture, tight serving loops).  Every
+// GPU holds its own copy of the array for the life of the process; its
+// device address is looked up once per device (hipGetSymbolAddress with
+// that device current) and only that address is passed to kernels.  Calls
+// on one GPU share the array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::logic_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    HIP_OK(hipGetDevice(&cur));
+    if (cur != device) HIP_OK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIP_OK(hipSetDevice(cur));
+    HIP_OK(e);
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the solve scratch");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+void Engine::solve_noalloc(const double* d_rhs, double* d_x, int nrhs) {
+  if (!factored_) throw std::logic_error("Solve called without a successful Factorize");
+  if (nrhs <= 0) throw std::invalid_argument("nrhs must be positive");
+  if (opt_.precision != 0)
+    throw std::logic_error("the allocation-free solve runs FP64 factors only (precision FP64)");
+  if (opt_.profile) throw std::logic_error("the allocation-free solve does not run in profile mode");
+  const size_t cnt = (size_t)n_ * (size_t)nrhs;
+  const size_t need = 3 * cnt * sizeof(double);
+  if (need > kNoAllocScratchBytes)
+    throw std::logic_error("allocation-free solve: 3 n nrhs doubles (" + std::to_string(need) +
+                           " bytes) exceed the static scratch of " + std::to_string(kNoAllocScratchBytes) +
+                           " bytes");
+  HIP_OK(hipSetDevice(opt_.device));
+  double* Y = (double*)noalloc_scratch(opt_.device);
+  double* X = Y + cnt;
+  double* Xf = X + cnt;
+  auto t0 = std::chrono::steady_clock::now();
+  ensure_ginv<double>(d_arena64_);
+  dev::k_gather_rhs<double><<<grid_rows64(n_), 256, 0, stream_>>>(d_rhs, Y, d_invp0_, n_, nrhs, nullptr);
+  solve_sequence<double>(d_arena64_, Y, X, nrhs, Xf);
+  dev::k_scatter_sol<double><<<grid_rows64(n_), 256, 0, stream_>>>(X, d_x, d_invp0_, n_, nrhs, nullptr, 0);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(stream_));
+  last_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  last_report = SolveReport();
+}
+
 void Engine::get_solution(double* B, int nrhs, bool on_device) {
   if (nrhs > cap_nrhs_) throw std::logic_error("GetSolution: nrhs larger than the last Solve");
   HIP_OK(hipMemcpyAsync(B, d_x_, (size_t)n_ * nrhs * sizeof(double),

@@ -15,6 +15,11 @@ namespace spk {
 
 constexpr int kRedBlocks = 64;  // blocks per column of the refinement reductions
 constexpr int kGmresRestart = 20;  // GMRES restart length of the LDL^T refinement
+// Size of the per-device array behind Engine::solve_noalloc: Y, X and Xf,
+// 3 n nrhs doubles, at most 128 MiB (n nrhs <= 5592405).
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+// Device address of that array on `device` (looked up once per device).
+void* noalloc_scratch(int device);
 
 struct EngineOptions {
   int ordering = ORD_MMD;
@@ -69,6 +74,11 @@ class Engine {
   int factorize();
   SolveReport solve(const double* rhs, int nrhs, bool on_device);
   void get_solution(double* B, int nrhs, bool on_device);
+  // FP64 solve of device-resident b into device-resident x that allocates
+  // nothing: work vectors in the static per-device scratch, kernels launched
+  // directly (no graph instantiation).  Rejects precision FP32_IR, profile
+  // mode and 3 n nrhs doubles beyond kNoAllocScratchBytes.
+  void solve_noalloc(const double* d_rhs, double* d_x, int nrhs);
 
   // HBM bytes this engine holds after DistributeMatrix (plan, maps, values,
   // factor) plus the solve buffers for nrhs right-hand sides.

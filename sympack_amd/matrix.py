@@ -266,6 +266,13 @@ class symPACKMatrix2D:
         _lib.check(self._lib.spk_solve_device(self._h, ctypes.c_void_p(dptr), int(nrhs)), "Solve")
         self._nrhs = int(nrhs)
 
+    def SolveDeviceNoAlloc(self, d_rhs, d_x, nrhs):
+        """Solve + GetSolution of an FP64 factor from device memory into
+        device memory without any device allocation (spk_solve_device_noalloc:
+        work vectors in the library's static 128 MiB per-device scratch)."""
+        _lib.check(self._lib.spk_solve_device_noalloc(self._h, ctypes.c_void_p(d_rhs), ctypes.c_void_p(d_x),
+                                                      int(nrhs)), "Solve")
+
     def GetSolution(self, nrhs=None):
         nrhs = self._nrhs if nrhs is None else nrhs
         out = np.zeros(self._n * nrhs, np.float64)
