@@ -191,8 +191,11 @@ def serving_leg(n_requests, rate, precision, slots, seed=0):
         rp.SetMaxSize(150)
         return sp.symPACKOptions(ordering="NDGRID" if dim == 2 else "NDBOX", relax=rp, precision=precision)
 
+    # HBM budget of the resident factors: 70% of the device memory free now
+    import torch
+    budget = int(0.7 * torch.cuda.mem_get_info()[0])
     futs = []
-    with SolverService(slots=slots) as svc:
+    with SolverService(hbm_budget_bytes=budget, slots=slots) as svc:
         t0 = time.perf_counter()
         j = 0
         for t, kind, key, p in stream:
@@ -212,6 +215,7 @@ def serving_leg(n_requests, rate, precision, slots, seed=0):
     rep["factor_requests"] = nf
     rep["arrival_rate_per_s"] = rate
     rep["slots"] = slots
+    rep["hbm_budget_gb"] = round(budget / 1e9, 1)
     rep["precision"] = precision
     return rep
 

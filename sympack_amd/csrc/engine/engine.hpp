@@ -15,6 +15,9 @@ namespace spk {
 
 constexpr int kRedBlocks = 64;  // blocks per column of the refinement reductions
 constexpr int kGmresRestart = 20;  // GMRES restart length of the LDL^T refinement
+// partitioned-inverse sweeps: workgroups a level launch aims at before it
+// splits its right-hand-side blocks over the grid
+constexpr int64_t kGinvGrid = 1024;
 // Size of the per-device array behind Engine::solve_noalloc: Y, X and Xf,
 // 3 n nrhs doubles, at most 128 MiB (n nrhs <= 5592405).
 constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
@@ -38,6 +41,9 @@ struct EngineOptions {
   std::vector<Int> user_perm;
 };
 
+// profile mask bit per kernel kind (LaunchRange::kind 0..12)
+constexpr unsigned kAllKinds = (1u << 13) - 1;
+
 struct KindStats {
   int64_t launches = 0;
   double ms = 0.0;        // summed event time (profile mode)
@@ -58,7 +64,7 @@ class Engine {
 
   void init(const EngineOptions& o);
   void set_user_perm(const std::vector<Int>& perm) { opt_.user_perm = perm; }
-  void set_profile(bool on, unsigned mask = 0x3ffu) {
+  void set_profile(bool on, unsigned mask = kAllKinds) {
     if ((on ? 1 : 0) != opt_.profile || mask != profile_mask_) destroy_graphs();
     opt_.profile = on ? 1 : 0;
     profile_mask_ = mask;
@@ -260,7 +266,7 @@ class Engine {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open_;
   std::vector<int64_t> ev_grid_;
   std::vector<int> ev_level_;
-  unsigned profile_mask_ = 0x3ffu;  // kinds whose launches get events
+  unsigned profile_mask_ = kAllKinds;  // kinds whose launches get events
   size_t ev_next_ = 0;
   void collect_events();
 };

@@ -1562,12 +1562,15 @@ __global__ __launch_bounds__(NT) void k_gprep(const SnInfo* __restrict__ sn, con
   }
 }
 
-// Forward level: rows [64 rt, 64 rt + 64) of [X_I; dY] = G_I Y_I for 16
-// right-hand sides (blockIdx.y), wave q owning 16 rows.  Every G and Y_I
-// operand is loaded before the first product (one memory round trip).
-// Rows a < w go to Xf (LDL: divided by D_a), rows below are subtracted from
-// Y at their structure rows with atomics.  Y_I of a level's supernodes is
-// never the target of the same level's updates (targets are ancestors).
+// Forward level: rows [64 rt, 64 rt + 64) of [X_I; dY] = G_I Y_I, wave q
+// owning 16 rows.  The right-hand sides go in blocks of 16: blocks
+// blockIdx.y, blockIdx.y + gridDim.y, ... (a level with few tiles spreads
+// the blocks over the grid; one with many walks them in one workgroup, so
+// the G tile is read from HBM once and stays in registers).  Every G and
+// Y_I operand of a block is loaded before its first product.  Rows a < w go
+// to Xf (LDL: divided by D_a), rows below are subtracted from Y at their
+// structure rows with atomics.  Y_I of a level's supernodes is never the
+// target of the same level's updates (targets are ancestors).
 template <typename T, bool LDL, int NBM>
 __global__ __launch_bounds__(NT) void k_gsolve_fwd(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks,
                                                    const T* __restrict__ garena, const T* __restrict__ arena,
@@ -1584,53 +1587,61 @@ __global__ __launch_bounds__(NT) void k_gsolve_fwd(const SnInfo* __restrict__ sn
   const int64_t ld = s.r;
   const int mi = lane & 15, g = lane >> 4;
   const int ac = min(a0 + mi, s.r - 1);
-  const int j0 = blockIdx.y * SUB;
-  const int jn = j0 + mi;
-  const int jc = min(jn, nrhs - 1);
-  const T jmask = T(jn < nrhs);
-  const T* __restrict__ Ga = garena + s.off + ac;           // G(a, k) at Ga[k ld]
-  const T* __restrict__ Yk = Y + (int64_t)s.fc * nrhs + jc;  // Y_I(k, j) at Yk[k nrhs]
-  T av[NQ], bv[NQ];
+  const T* __restrict__ Ga = garena + s.off + ac;  // G(a, k) at Ga[k ld]
+  T av[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int k = 4 * q + g;
-    const int kc = min(k, w - 1);
-    av[q] = Ga[(int64_t)kc * ld];
-    bv[q] = Yk[(int64_t)kc * nrhs];
+    av[q] = Ga[(int64_t)min(k, w - 1) * ld] * T(k < w);
   }
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    keep(av[q]);
-    keep(bv[q]);
-  }
-  v4 acc = v4{0, 0, 0, 0};
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    if (4 * q < w) {  // uniform: chunks wholly beyond w are skipped
-      const T km = T(4 * q + g < w);
-      acc = Mfma<T>::mma(av[q] * km, bv[q] * jmask, acc);
-    }
-  }
-  if (jn >= nrhs) return;
+  // output rows of this lane: structure rows below the block, or -1
+  int orow[4];
+  T dinv[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int a = a0 + Mfma<T>::row(lane, v);
-    if (a < w) {
-      T x = acc[v];
-      if (LDL) x /= arena[s.off + (int64_t)a * ld + a];
-      Xf[(int64_t)(s.fc + a) * nrhs + jn] = x;
-    } else if (a < s.r) {
-      atomic_add(Y + (int64_t)lindx0[s.lx + a] * nrhs + jn, -acc[v]);
+    orow[v] = (a >= w && a < s.r) ? lindx0[s.lx + a] : -1;
+    dinv[v] = T(1);
+    if (LDL && a < w) dinv[v] = T(1) / arena[s.off + (int64_t)a * ld + a];
+  }
+  const int nblk = (nrhs + SUB - 1) / SUB;
+  for (int jb = blockIdx.y; jb < nblk; jb += gridDim.y) {
+    const int jn = jb * SUB + mi;
+    const int jc = min(jn, nrhs - 1);
+    const T jmask = T(jn < nrhs);
+    const T* __restrict__ Yk = Y + (int64_t)s.fc * nrhs + jc;  // Y_I(k, j) at Yk[k nrhs]
+    T bv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) bv[q] = Yk[(int64_t)min(4 * q + g, w - 1) * nrhs];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) keep(bv[q]);
+    v4 acc = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (4 * q < w)  // uniform: chunks wholly beyond w are skipped
+        acc = Mfma<T>::mma(av[q], bv[q] * jmask, acc);
+    }
+    if (jn < nrhs) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int a = a0 + Mfma<T>::row(lane, v);
+        if (a < w)
+          Xf[(int64_t)(s.fc + a) * nrhs + jn] = LDL ? acc[v] * dinv[v] : acc[v];
+        else if (orow[v] >= 0)
+          atomic_add(Y + (int64_t)orow[v] * nrhs + jn, -acc[v]);
+      }
     }
   }
 }
 
-// Backward level: X_I += G_I(tile rows, :)^T V(tile rows) for 16 right-hand
-// sides, V(a) = Xf(a) on the diagonal rows and -X(struct row a) below (the
-// ancestors' final values).  Lane (m, g) reads rows 16g..16g+15 of one G
-// column (a cache line), wave q the 16-column blocks q, q+4, q+8; every load
-// is issued before the first product; partial sums of the tile go to X_I
-// with atomics (X zeroed before the sweep).
+// Backward level: X_I += G_I(tile rows, :)^T V(tile rows), V(a) = Xf(a) on
+// the diagonal rows and -X(struct row a) below (the ancestors' final
+// values).  Lane (m, g) reads rows 16g..16g+15 of one G column (a cache
+// line), wave q the 16-column blocks q, q+4, q+8; the G values stay in
+// registers while the workgroup walks its right-hand-side blocks (as in the
+// forward level); every load of a block is issued before its first product;
+// partial sums of the tile go to X_I with atomics (X zeroed before the
+// sweep).
 template <typename T, int NBM>
 __global__ __launch_bounds__(NT) void k_gsolve_bwd(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks,
                                                    const T* __restrict__ garena,
@@ -1646,11 +1657,6 @@ __global__ __launch_bounds__(NT) void k_gsolve_bwd(const SnInfo* __restrict__ sn
   if (wave >= nkb) return;
   const int a0 = t.b * TILE;
   const int mi = lane & 15, g = lane >> 4;
-  const int j0 = blockIdx.y * SUB;
-  const int jn = j0 + mi;
-  const int jc = min(jn, nrhs - 1);
-  const T jmask = T(jn < nrhs);
-  const int64_t ld = s.r;
   int rowi[16];
 #pragma unroll
   for (int st = 0; st < 16; ++st) rowi[st] = lindx0[s.lx + min(a0 + 16 * g + st, s.r - 1)];
@@ -1658,34 +1664,38 @@ __global__ __launch_bounds__(NT) void k_gsolve_bwd(const SnInfo* __restrict__ sn
 #pragma unroll
   for (int i = 0; i < MKB; ++i) {
     const int kcol = min((wave + 4 * i) * 16 + mi, w - 1);
-    const T* __restrict__ Gc = garena + s.off + (int64_t)kcol * ld;
+    const T* __restrict__ Gc = garena + s.off + (int64_t)kcol * s.r;
 #pragma unroll
     for (int st = 0; st < 16; ++st) av[i][st] = Gc[min(a0 + 16 * g + st, s.r - 1)];
   }
-  T bv[16];
+  const int nblk = (nrhs + SUB - 1) / SUB;
+  for (int jb = blockIdx.y; jb < nblk; jb += gridDim.y) {
+    const int jn = jb * SUB + mi;
+    const int jc = min(jn, nrhs - 1);
+    const T jmask = T(jn < nrhs);
+    T bv[16];
 #pragma unroll
-  for (int st = 0; st < 16; ++st) {
-    const int a = a0 + 16 * g + st;
-    const T* src = a < w ? Xf : X;
-    const T sg = a < w ? T(1) : T(-1);
-    bv[st] = src[(int64_t)rowi[st] * nrhs + jc] * (jmask * T(a < s.r) * sg);
-  }
+    for (int st = 0; st < 16; ++st) {
+      const int a = a0 + 16 * g + st;
+      const T* src = a < w ? Xf : X;
+      const T sg = a < w ? T(1) : T(-1);
+      bv[st] = src[(int64_t)rowi[st] * nrhs + jc] * (jmask * T(a < s.r) * sg);
+    }
 #pragma unroll
-  for (int i = 0; i < MKB; ++i)
+    for (int st = 0; st < 16; ++st) keep(bv[st]);
 #pragma unroll
-    for (int st = 0; st < 16; ++st) keep(av[i][st]);
+    for (int i = 0; i < MKB; ++i) {
+      const int kb = wave + 4 * i;
+      if (kb < nkb) {
+        v4 acc = v4{0, 0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < MKB; ++i) {
-    const int kb = wave + 4 * i;
-    if (kb < nkb) {
-      v4 acc = v4{0, 0, 0, 0};
+        for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[i][st], bv[st], acc);
+        if (jn < nrhs) {
 #pragma unroll
-      for (int st = 0; st < 16; ++st) acc = Mfma<T>::mma(av[i][st], bv[st], acc);
-      if (jn < nrhs) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int k = kb * 16 + Mfma<T>::row(lane, v);
-          if (k < w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, acc[v]);
+          for (int v = 0; v < 4; ++v) {
+            const int k = kb * 16 + Mfma<T>::row(lane, v);
+            if (k < w) atomic_add(X + (int64_t)(s.fc + k) * nrhs + jn, acc[v]);
+          }
         }
       }
     }
