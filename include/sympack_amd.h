@@ -56,7 +56,8 @@ typedef struct spk_options {
   double zrelax0, zrelax1, zrelax2; /* 0.8, 0.1, 0.05 */
   int precision;      /* SPK_PREC_*; default FP64 */
   int ir_max_iter;    /* refinement steps for SPK_PREC_FP32_IR (default 30): preconditioned
-                         conjugate gradients for LL^T, fixed-point refinement for LDL^T */
+                         conjugate gradient iterations for LL^T, GMRES steps (restarted every
+                         20) for static LDL^T */
   double ir_tol;      /* normwise backward-error target (default 1e-12) */
   const int* perm;    /* SPK_ORDER_USER: perm[new] = old, any base (symPACKOptions.perm) */
   int verbose;
@@ -70,9 +71,10 @@ typedef struct spk_options {
                          The dependency-driven sweeps (one persistent launch each) are
                          disabled in this build; Solve rejects 1 */
   int refine;         /* supernode reordering (symPACKOptions.order_refinement_str,
-                         symPACKMatrix2D.hpp:3090-3111): 0 = NO (default), or
+                         symPACKMatrix2D.hpp:3090-3263): 0 = NO (default), or
                          10*ordflag + altflag: 10, 11, 20, 21, 30, 31 for SET10 .. SET31
-                         (SET = 31) */
+                         (SET = 31), 40 = TSPB (traveling-salesman tour with skeleton
+                         compression), 41 = any other TSPB* name (tour without it) */
   int solve_inverse;  /* 1 (default): where every supernode is at most 160 columns wide, the
                          solve uses the partitioned inverse G_I = [I; L_off] L_II^{-1} of each
                          supernode (prepared once per factorization), one launch per etree level

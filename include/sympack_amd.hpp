@@ -61,10 +61,13 @@ inline int ordering_code(const std::string& s) {
 }
 
 /* options.order_refinement_str -> spk_options.refine
- * (symPACKMatrix2D.hpp:3090-3111): SET is SET31; names the reference does
- * not act on leave the ordering unchanged.  TSPB is not provided. */
+ * (symPACKMatrix2D.hpp:3090-3263): SET is SET31; TSPB is the tour with
+ * skeleton compression (40), any other name starting with TSPB the tour
+ * without it (41); names the reference does not act on leave the ordering
+ * unchanged. */
 inline int refine_code(const std::string& r) {
-  if (r.compare(0, 4, "TSPB") == 0) throw std::logic_error("-refine TSPB is not supported; use SET, SET10 .. SET31");
+  if (r == "TSPB") return 40;
+  if (r.compare(0, 4, "TSPB") == 0) return 41;
   if (r == "SET" || r == "SET31") return 31;
   if (r == "SET10") return 10;
   if (r == "SET11") return 11;

@@ -118,15 +118,30 @@ struct Symbolic {
   Int nsuper() const { return (Int)xsuper.size() - 1; }
 };
 
+constexpr int kRefineTspb = 40;       // TSPB: tour with skeleton compression
+constexpr int kRefineTspbPlain = 41;  // any other TSPB* name: without compression
+
 // Full analysis as in symPACKMatrix2D::SymbolicFactorization
 // (include/sympack/symPACKMatrix2D.hpp:2699-3081).  The input is the
 // caller's lower-triangular CSC pattern (incl. diagonal).
 // user_perm: for ORD_USER only (any base; shifted by its minimum).
 // refine: 0, or 10*ordflag + altflag of the supernode reordering
-// (-refine SET10 .. SET31; symPACKMatrix2D.hpp:3090-3111).
+// (-refine SET10 .. SET31; symPACKMatrix2D.hpp:3090-3111), or
+// kRefineTspb / kRefineTspbPlain (-refine TSPB / other TSPB* names,
+// symPACKMatrix2D.hpp:3113-3263).
 void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int baseval,
                             int ordering, const RelaxParams& relax, const Int* user_perm,
                             Symbolic& out, int refine = 0);
+
+// Traveling-salesman reordering inside the supernodes (tspb.cpp; -refine
+// TSPB with skeleton compression, other TSPB* names without): new position
+// of every column (1-based), to be composed into the ordering.  graph: the
+// expanded pattern in the original numbering; et, ord: the postordered tree
+// and the ordering the structure was built with.
+std::vector<Int> refine_supernodes_tsp(bool compress, Int n, const Graph& graph, const ETree& et,
+                                       const Ordering& ord, const std::vector<Int>& xsuper,
+                                       const std::vector<Int>& snode, const std::vector<Ptr>& xlindx,
+                                       const std::vector<Idx>& lindx);
 
 // Reordering inside the supernodes (refine.cpp): new position of every
 // column (1-based), to be composed into the ordering.

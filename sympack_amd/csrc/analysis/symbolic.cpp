@@ -435,8 +435,12 @@ void symbolic_factorization(Int n, const Ptr* colptr, const Idx* rowind, int bas
     // symPACKMatrix2D.hpp:3090-3111: reorder inside the supernodes, compose
     // (Ordering::Compose, src/Ordering.cpp:1301-1314), and build the row
     // structure again with the same partition (3246-3254)
-    const std::vector<Int> invp2 = refine_supernodes(refine / 10, refine % 10, n, out.xsuper,
-                                                     out.supMembership, out.xlindx, out.lindx);
+    const std::vector<Int> invp2 =
+        (refine == kRefineTspb || refine == kRefineTspbPlain)
+            ? refine_supernodes_tsp(refine == kRefineTspb, n, graph, et, ord, out.xsuper, out.supMembership,
+                                    out.xlindx, out.lindx)
+            : refine_supernodes(refine / 10, refine % 10, n, out.xsuper, out.supMembership, out.xlindx,
+                                out.lindx);
     for (Int i = 0; i < n; ++i) ord.invp[i] = invp2[ord.invp[i] - 1];
     for (Int i = 0; i < n; ++i) ord.perm[ord.invp[i] - 1] = i + 1;
     symbolic_structure(graph, ord, et, cc, out.xsuper, out.supMembership, first_col_only,

@@ -77,8 +77,8 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   // the reordering codes ordsup takes: 10 * ordflag + altflag
   // (symPACKMatrix2D.hpp:3090-3111); anything else is an argument error
   if (o.refine != 0 && o.refine != 10 && o.refine != 11 && o.refine != 20 && o.refine != 21 &&
-      o.refine != 30 && o.refine != 31)
-    throw std::logic_error("refine must be 0, 10, 11, 20, 21, 30 or 31, got " + std::to_string(o.refine));
+      o.refine != 30 && o.refine != 31 && o.refine != spk::kRefineTspb && o.refine != spk::kRefineTspbPlain)
+    throw std::logic_error("refine must be 0, 10, 11, 20, 21, 30, 31, 40 or 41, got " + std::to_string(o.refine));
   if (o.precision != SPK_PREC_FP64 && o.precision != SPK_PREC_FP32_IR)
     throw std::logic_error("precision must be SPK_PREC_FP64 or SPK_PREC_FP32_IR");
   e.refine = o.refine;
@@ -204,8 +204,9 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
       } else if (k == "-merge") {
         o->merge_updates = std::atoi(val(1).c_str());
       } else if (k == "-refine") {
-        // symPACKMatrix2D.hpp:3090-3111 (utils.hpp:331-333): SET = SET31;
-        // other names leave the ordering as it is
+        // symPACKMatrix2D.hpp:3090-3263 (utils.hpp:331-333): SET = SET31,
+        // TSPB = tour with skeleton compression, other TSPB* names = tour
+        // without it; other names leave the ordering as it is
         std::string v = val(1);
         if (v == "SET" || v == "SET31") o->refine = 31;
         else if (v == "SET10") o->refine = 10;
@@ -213,8 +214,8 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         else if (v == "SET20") o->refine = 20;
         else if (v == "SET21") o->refine = 21;
         else if (v == "SET30") o->refine = 30;
-        else if (v.compare(0, 4, "TSPB") == 0)
-          throw std::logic_error("-refine TSPB is not supported; use SET, SET10 .. SET31");
+        else if (v == "TSPB") o->refine = spk::kRefineTspb;
+        else if (v.compare(0, 4, "TSPB") == 0) o->refine = spk::kRefineTspbPlain;
         else o->refine = 0;
       } else if (k == "-solve_inverse") {
         o->solve_inverse = std::atoi(val(1).c_str()) != 0;

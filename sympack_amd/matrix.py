@@ -44,6 +44,28 @@ class RelaxationParameters:
             self.nrelax2 = min(self.nrelax2, m)
 
 
+def refine_code(name):
+    """symPACKOptions.order_refinement_str -> spk_options.refine
+    (symPACKMatrix2D.hpp:3090-3263): SET = SET31, SET10 .. SET31 = 10*ordflag
+    + altflag, TSPB = 40 (tour with skeleton compression), other TSPB* names
+    = 41 (without it), anything else = 0 (ordering left as it is)."""
+    if name == "TSPB":
+        return 40
+    if name.startswith("TSPB"):
+        return 41
+    return {"SET": 31, "SET10": 10, "SET11": 11, "SET20": 20, "SET21": 21,
+            "SET30": 30, "SET31": 31}.get(name, 0)
+
+
+def refine_name(code):
+    """spk_options.refine -> an order_refinement_str with that meaning."""
+    if code == 40:
+        return "TSPB"
+    if code == 41:
+        return "TSPB1"
+    return "SET%d" % code if code else "NO"
+
+
 class symPACKOptions:
     """The symPACKOptions fields run_sympack2D sets (driver/utils.hpp:227-452),
     with the driver's defaults: MMD, LL, relax (8,32,64), unlimited width."""
@@ -64,8 +86,9 @@ class symPACKOptions:
         self.device = device
         self.merge_updates = merge_updates
         self.solve_dataflow = solve_dataflow
-        # supernode reordering (symPACKMatrix2D.hpp:3090-3111): NO, SET,
-        # SET10, SET11, SET20, SET21, SET30, SET31
+        # supernode reordering (symPACKMatrix2D.hpp:3090-3263): NO, SET,
+        # SET10, SET11, SET20, SET21, SET30, SET31, TSPB (tour with skeleton
+        # compression), other TSPB* names (tour without it)
         self.order_refinement_str = order_refinement_str
         # partitioned-inverse solve where every supernode is <= 160 wide
         self.solve_inverse = solve_inverse
@@ -85,7 +108,7 @@ class symPACKOptions:
                    relax=r, precision="FP32_IR" if o.precision else "FP64",
                    ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
                    merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow),
-                   order_refinement_str="SET%d" % o.refine if o.refine else "NO",
+                   order_refinement_str=refine_name(o.refine),
                    solve_inverse=bool(o.solve_inverse))
 
     def to_c(self):
@@ -110,11 +133,7 @@ class symPACKOptions:
         o.merge_updates = int(self.merge_updates)
         o.solve_dataflow = 1 if self.solve_dataflow else 0
         o.solve_inverse = 1 if self.solve_inverse else 0
-        r = self.order_refinement_str
-        if r.startswith("TSPB"):
-            raise ValueError("-refine TSPB is not supported; use SET, SET10 .. SET31")
-        o.refine = {"SET": 31, "SET10": 10, "SET11": 11, "SET20": 20, "SET21": 21,
-                    "SET30": 30, "SET31": 31}.get(r, 0)
+        o.refine = refine_code(self.order_refinement_str)
         return o
 
 

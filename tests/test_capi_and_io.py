@@ -65,10 +65,10 @@ def test_refine_code_outside_the_reordering_modes_is_rejected():
     try:
         o = _lib.SpkOptions()
         lib.spk_default_options(ctypes.byref(o))
-        for good in (0, 10, 11, 20, 21, 30, 31):
+        for good in (0, 10, 11, 20, 21, 30, 31, 40, 41):
             o.refine = good
             assert lib.spk_init(h, ctypes.byref(o)) == 0
-        for bad in (12, 40, 1, -5):
+        for bad in (12, 42, 50, 1, -5):
             o.refine = bad
             assert lib.spk_init(h, ctypes.byref(o)) == -2
             assert b"refine" in lib.spk_last_error()

@@ -1,4 +1,5 @@
-"""GPU factor and solve on a reordered supernode numbering (-refine SET*):
+"""GPU factor and solve on a reordered supernode numbering (-refine SET*,
+TSPB*):
 the HIP path against the oracle on the same refined structure, and against
 the unrefined run of the same system.  Tolerances as in the other parity
 tests: panels and solutions within 1e-10 relative of the oracle's."""
@@ -35,6 +36,9 @@ def _solve(gen, ordering, refine, ldl, nrhs=3, seed=0):
     ("diff3d_14_amd_set21", lambda: M.diffusion_3d_7pt(14, seed=3), "AMD", "SET21", False),
     ("lap2d_48_mmd_set10", lambda: M.laplacian_2d_5pt(48), "MMD", "SET10", False),
     ("diff3d_12_mmd_set_ldl", lambda: M.diffusion_3d_7pt(12, seed=5), "MMD", "SET", True),
+    ("st27_8_mmd_tspb", lambda: M.stencil_3d_27pt(8, seed=2), "MMD", "TSPB", False),
+    ("lap2d_48_amd_tspb1", lambda: M.laplacian_2d_5pt(48), "AMD", "TSPB1", False),
+    ("diff3d_12_mmd_tspb_ldl", lambda: M.diffusion_3d_7pt(12, seed=5), "MMD", "TSPB", True),
 ])
 def test_refined_factor_and_solve_match_oracle(name, gen, ordering, code, ldl):
     A, Mx, X, B = _solve(gen, ordering, code, ldl)
