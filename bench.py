@@ -379,6 +379,14 @@ def main():
         M.Factorize()
         M.SolveDevice(b.data_ptr(), 1)
 
+    # the autotuner ran on the first (warmup) factorization: per-level Schur
+    # launch configurations timed on the device
+    tl = M.autotune_log()
+    autotune = {"levels": len(tl), "changed": sum(1 for r in tl if r["ms_best"] < r["ms_default"]),
+                "schur_ms_default": round(sum(r["ms_default"] for r in tl), 4),
+                "schur_ms_tuned": round(sum(r["ms_best"] for r in tl), 4),
+                "choices": [[r["level"], r["split"], r["order"]] for r in tl]} if tl else None
+
     # timed region 1: K steps replayed from the captured HIP graphs
     M.reset_kernel_stats()
     torch.cuda.synchronize()
@@ -531,6 +539,7 @@ def main():
                      "traffic_source": "profiles/schur_update_hbm_traffic_48cube.md" if traffic else None,
                      "flops_per_launch": schur_flops_per_launch,
                      "avg_launch_ms": schur_avg_ms},
+        "autotune": autotune,
         "cpu_baseline": cpu,
         "precision_fp32_ir_vs_fp64": prec,
         "ldl_indefinite": ldl,

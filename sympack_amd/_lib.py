@@ -37,6 +37,8 @@ class SpkOptions(ctypes.Structure):
         ("solve_dataflow", ctypes.c_int),
         ("refine", ctypes.c_int),
         ("solve_inverse", ctypes.c_int),
+        ("autotune", ctypes.c_int),
+        ("schur_tile", ctypes.c_int),
     ]
 
 
@@ -101,6 +103,7 @@ SIGNATURES = [
     ("spk_release_device", _I, [_P]),
     ("spk_get_launch_trace", _I, [_P, _PI32, _PI64, _PD, _I]),
     ("spk_get_launch_levels", _I, [_P, _PI32, _I]),
+    ("spk_get_autotune", _I, [_P, _PI32, _PI32, _PI32, _PD, _PD, _I]),
     ("spk_get_supernode_levels", _I, [_P, _PI32]),
     ("spk_get_supernodal_partition", _I, [_P, _PI32]),
     ("spk_get_sup_membership", _I, [_P, _PI32]),

@@ -37,8 +37,12 @@ Graph expand_symmetric(Int n, const Ptr* colptr, const Idx* rowind, int baseval)
   for (Int c = 0; c < n; ++c)
     for (Ptr p = colptr[c] - baseval; p < colptr[c + 1] - baseval; ++p)
       g.rowind[pos[c]++] = rowind[p] - baseval + 1;
-  for (Int c = 0; c < n; ++c)
-    std::sort(g.rowind.begin() + (g.colptr[c] - 1), g.rowind.begin() + (g.colptr[c + 1] - 1));
+  // a column holds its mirrored rows (< its own index, ascending) and then
+  // its own lower rows: sorted as soon as the input column is
+  for (Int c = 0; c < n; ++c) {
+    auto b = g.rowind.begin() + (g.colptr[c] - 1), e = g.rowind.begin() + (g.colptr[c + 1] - 1);
+    if (!std::is_sorted(b, e)) std::sort(b, e);
+  }
   g.keep_diag = true;
   return g;
 }

@@ -83,6 +83,10 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
     throw std::logic_error("precision must be SPK_PREC_FP64 or SPK_PREC_FP32_IR");
   e.refine = o.refine;
   e.solve_inverse = o.solve_inverse;
+  e.autotune = o.autotune;
+  if (o.schur_tile != 0 && o.schur_tile != 64 && o.schur_tile != 128)
+    throw std::logic_error("schur_tile must be 0, 64 or 128, got " + std::to_string(o.schur_tile));
+  e.schur_tile = o.schur_tile;
   return e;
 }
 }  // namespace
@@ -141,6 +145,8 @@ void spk_default_options(spk_options* o) {
   o->solve_dataflow = 0;
   o->refine = 0;
   o->solve_inverse = 1;
+  o->autotune = 1;
+  o->schur_tile = 0;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -219,6 +225,10 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         else o->refine = 0;
       } else if (k == "-solve_inverse") {
         o->solve_inverse = std::atoi(val(1).c_str()) != 0;
+      } else if (k == "-autotune") {
+        o->autotune = std::atoi(val(1).c_str()) != 0;
+      } else if (k == "-schur_tile") {
+        o->schur_tile = std::atoi(val(1).c_str());
       } else if (k == "-dataflow") {
         o->solve_dataflow = std::atoi(val(1).c_str()) != 0;
       }
@@ -427,6 +437,21 @@ int spk_get_launch_levels(spk_handle h, int32_t* levels, int max) {
   const auto& t = h->engine.trace;
   int cnt = (int)std::min<size_t>(t.size(), (size_t)std::max(0, max));
   for (int i = 0; i < cnt; ++i) levels[i] = t[i].level;
+  return cnt;
+}
+
+int spk_get_autotune(spk_handle h, int32_t* level, int32_t* split, int32_t* order, double* ms_default,
+                     double* ms_best, int max) {
+  if (!h || !level || !split || !order || !ms_default || !ms_best) return fail("null argument");
+  const auto& t = h->engine.tune_log;
+  int cnt = (int)std::min<size_t>(t.size(), (size_t)std::max(0, max));
+  for (int i = 0; i < cnt; ++i) {
+    level[i] = t[i].level;
+    split[i] = t[i].split;
+    order[i] = t[i].alt;
+    ms_default[i] = t[i].ms0;
+    ms_best[i] = t[i].ms;
+  }
   return cnt;
 }
 

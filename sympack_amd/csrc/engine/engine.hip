@@ -51,7 +51,7 @@ inline unsigned grid_for(int64_t n, int block = 256) {
 }
 // k_gather_rhs / k_scatter_sol: one workgroup per 64 rows
 inline unsigned grid_rows64(int n) { return (unsigned)std::max(1, (n + 63) / 64); }
-enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED, TA_GINV };
+enum TaskArr { TA_POTRF = 0, TA_TRSM, TA_INPANEL, TA_SCHUR, TA_SDIAG, TA_SOFF, TA_SOFFM, TA_FUSED, TA_GINV, TA_SCHUR_LPT, TA_WIDE, TA_WIDE_LPT };
 }  // namespace
 
 namespace {
@@ -338,7 +338,7 @@ int64_t Engine::device_bytes(int nrhs) const {
   auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
   const Plan& p = plan_;
   int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + p.rel_size * 4 + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
-              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + vb(p.schur_tasks) + vb(p.fused_tasks) +
+              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + 2 * vb(p.schur_tasks) + 2 * vb(p.schur_wide) + vb(p.fused_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   b += (int64_t)n_ * 4 + 4;                                            // invp0, pivot flag
@@ -372,6 +372,8 @@ void Engine::symbolic(Int n, const Ptr* colptr, const Idx* rowind, int baseval) 
   symbolic_factorization(n, colptr, rowind, baseval, opt_.ordering, opt_.relax, up, sym_,
                          opt_.refine);
   build_plan(sym_, n, colptr, rowind, baseval, plan_, opt_.merge_updates);
+  tuned_ = false;
+  tune_log.clear();
   n_ = n;
   nnzA_ = colptr[n] - baseval;
   h_colptr_.assign(n + 1, 0);
@@ -430,6 +432,9 @@ void Engine::ensure_device() {
   d_tasks_[TA_TRSM] = dupload(plan_.trsm_tasks, s);
   d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
   d_tasks_[TA_SCHUR] = dupload(plan_.schur_tasks, s);
+  d_tasks_[TA_SCHUR_LPT] = dupload(plan_.schur_tasks_lpt, s);
+  d_tasks_[TA_WIDE] = dupload(plan_.schur_wide, s);
+  d_tasks_[TA_WIDE_LPT] = dupload(plan_.schur_wide_lpt, s);
   d_tasks_[TA_FUSED] = dupload(plan_.fused_tasks, s);
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
   d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
@@ -442,13 +447,16 @@ void Engine::ensure_device() {
     HIP_OK(hipMemsetAsync(d_fcnt_, 0, sizeof(int) * (size_t)plan_.nsuper, s));
   }
   if (plan_.df_ok && use_df()) {  // switched off: nothing is uploaded
-    d_df_fwd_ = dupload(plan_.df_fwd, s);
-    d_df_bwd_ = dupload(plan_.df_bwd, s);
-    d_df_fend_ = dupload(plan_.df_fwd_end, s);
-    d_df_bend_ = dupload(plan_.df_bwd_end, s);
-    d_df_tgt_ = dupload(plan_.df_tgt, s);
-    d_df_dcnt0_ = dupload(plan_.df_dcnt0, s);
-    d_df_rcnt0_ = dupload(plan_.df_rcnt0, s);
+    const DataflowLists df = build_dataflow_lists(plan_);
+    df_nf_ = (int)df.fwd.size();
+    df_nb_ = (int)df.bwd.size();
+    d_df_fwd_ = dupload(df.fwd, s);
+    d_df_bwd_ = dupload(df.bwd, s);
+    d_df_fend_ = dupload(df.fwd_end, s);
+    d_df_bend_ = dupload(df.bwd_end, s);
+    d_df_tgt_ = dupload(df.tgt, s);
+    d_df_dcnt0_ = dupload(df.dcnt0, s);
+    d_df_rcnt0_ = dupload(df.rcnt0, s);
     d_invb_ = dupload(plan_.inv_base, s);
     d_df_state_ = dalloc<int>(3 + 4 * (size_t)plan_.nsuper);
     int cus = 0;
@@ -801,6 +809,33 @@ void Engine::launch_diagtrsm(int bucket, bool ldl, unsigned g, const Task4* t, T
 #undef SPK_FUSED
 }
 
+// One Schur launch of range r: 64 x 64 (k_update) or 128 x 64
+// (k_schur_wide) tiles (alt bit 1), plan or longest-first task order (alt
+// bit 0), each tile's k chunks split over ks workgroups.
+template <typename T>
+void Engine::launch_schur(const LaunchRange& r, int alt, int ks, T* arena, hipStream_t st) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const bool ldl = opt_.ldl != 0;
+  const bool wide = (alt & 2) != 0;
+  const int64_t cnt = wide ? r.count_w : r.count;
+  if (cnt == 0) return;
+  const Task4* t = (const Task4*)d_tasks_[wide ? ((alt & 1) ? TA_WIDE_LPT : TA_WIDE)
+                                              : ((alt & 1) ? TA_SCHUR_LPT : TA_SCHUR)] +
+                   (wide ? r.first_w : r.first);
+  const unsigned gs = (unsigned)(cnt * ks);
+  if (wide) {
+    if (ldl)
+      dev::k_schur_wide<T, true><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+    else
+      dev::k_schur_wide<T, false><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+  } else {
+    if (ldl)
+      dev::k_update<T, 1, true><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+    else
+      dev::k_update<T, 1, false><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+  }
+}
+
 template <typename T>
 void Engine::factor_sequence(T* arena) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
@@ -833,22 +868,96 @@ void Engine::factor_sequence(T* arena) {
           dev::k_update<T, 0, false><<<g, 256, 0, stream_>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, 1);
         break;
       }
-      case 2: {
-        const Task4* t = (const Task4*)d_tasks_[TA_SCHUR] + r.first;
-        const int ks = std::max(1, r.aux);  // k-chunk split: one workgroup per (task, part)
-        const unsigned gs = g * (unsigned)ks;
-        if (ldl)
-          dev::k_update<T, 1, true><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, ks);
-        else
-          dev::k_update<T, 1, false><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, r.count, ks);
+      case 2:
+        launch_schur<T>(r, r.alt, std::max(1, r.aux), arena, st);
         break;
-      }
       default:
         throw std::logic_error("bad factor launch kind");
     }
     launch_end(r.kind, st);
   }
   HIP_OK(hipGetLastError());
+}
+
+// Autotuner of the Schur launches (one per etree level).  Candidates: the
+// plan's task order or longest-first, times a k-chunk split of 1, 2, 3, 4, 6
+// or 8 workgroups per tile (at most the fewest chunks of any tile of the
+// range, and at most 8192 workgroups).  Each candidate is launched three
+// times on the arena, timed with HIP events on the library's stream, and the
+// fastest minimum wins; the default configuration keeps its place unless
+// another is at least 3% faster.  The Schur kernel only reads panels and
+// adds into the arena, so the launches leave garbage that the
+// factorization's assembly (fill + scatter of A) overwrites.
+template <typename T>
+void Engine::autotune_schur(T* arena) {
+  const SnInfo* sn = (const SnInfo*)d_sn_;
+  const bool ldl = opt_.ldl != 0;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } guard{e0, e1};
+  auto launch = [&](const LaunchRange& r, int alt, int ks) { launch_schur<T>(r, alt, ks, arena, stream_); };
+  auto time_of = [&](const LaunchRange& r, int alt, int ks) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      HIP_OK(hipEventRecord(e0, stream_));
+      launch(r, alt, ks);
+      HIP_OK(hipEventRecord(e1, stream_));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, ms);
+    }
+    HIP_OK(hipGetLastError());
+    return (double)best;
+  };
+  tune_log.clear();
+  bool warm = false;
+  for (LaunchRange& r : plan_.factor_seq) {
+    if (r.kind != 2 || r.count == 0) continue;
+    int nqmin = INT_MAX;
+    for (int64_t q = r.first; q < r.first + r.count; ++q) {
+      const Task4& t = plan_.schur_tasks[q];
+      int nq = (plan_.sn[t.a].w + kSchurKC - 1) / kSchurKC;
+      if (t.d > 0) nq += (plan_.merges[t.d - 1].w + kSchurKC - 1) / kSchurKC;
+      nqmin = std::min(nqmin, nq);
+    }
+    if (!warm) {  // first launches of the kernels in this process: load them untimed
+      launch(r, 0, std::max(1, r.aux));
+      if (r.count_w > 0) launch(r, 2, 1);
+      warm = true;
+    }
+    TuneRecord rec;
+    rec.level = r.level;
+    rec.tiles = r.count;
+    rec.split0 = rec.split = std::max(1, r.aux);
+    rec.alt0 = rec.alt = r.alt;
+    rec.ms0 = rec.ms = time_of(r, r.alt, rec.split0);
+    rec.candidates = 1;
+    for (int alt = 0; alt < 4; ++alt)
+      for (int ks : {1, 2, 3, 4, 6, 8}) {
+        const int64_t tiles = (alt & 2) ? r.count_w : r.count;
+        if (tiles == 0 || ks > nqmin || tiles * ks > 8192) continue;
+        if (alt == rec.alt0 && ks == rec.split0) continue;
+        const double ms = time_of(r, alt, ks);
+        ++rec.candidates;
+        if (ms < rec.ms && ms < 0.97 * rec.ms0) {
+          rec.ms = ms;
+          rec.split = ks;
+          rec.alt = alt;
+        }
+      }
+    r.aux = rec.split;
+    r.alt = rec.alt;
+    tune_log.push_back(rec);
+  }
+  HIP_OK(hipStreamSynchronize(stream_));
 }
 
 int Engine::factorize() {
@@ -858,6 +967,21 @@ int Engine::factorize() {
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
   ginv_ready_ = false;
+  if (opt_.schur_tile != 0 && !tuned_) {
+    // a fixed tile shape for every Schur launch, plan order and k split
+    for (LaunchRange& r : plan_.factor_seq)
+      if (r.kind == 2) r.alt = (opt_.schur_tile == 128 && r.count_w > 0) ? 2 : 0;
+    tuned_ = true;
+    destroy_graphs();
+  }
+  if (opt_.autotune && !tuned_ && plan_.schur_flops >= kAutotuneMinFlops) {
+    if (opt_.precision == 0)
+      autotune_schur<double>(d_arena64_);
+    else
+      autotune_schur<float>(d_arena32_);
+    tuned_ = true;
+    destroy_graphs();  // the captured factor sequence holds the old configuration
+  }
   if (opt_.precision == 0) {
     run_graph(fgraph_, 1, [&] {
       assemble(d_arena64_);
@@ -971,7 +1095,7 @@ void Engine::solve_sequence_df(const T* arena, T* Y, T* X, int nrhs) {
   st.bflag = st.rcnt + ns;
   st.abort_ = st.bflag + ns;
   dev::k_df_reset<<<grid_for(ns), 256, 0, stream_>>>(st, d_df_dcnt0_, d_df_rcnt0_, ns);
-  const int nf = (int)plan_.df_fwd.size(), nb = (int)plan_.df_bwd.size();
+  const int nf = df_nf_, nb = df_nb_;
   launch_begin(3, nf);
   dev::k_fsolve_df<T><<<(unsigned)std::min(df_grid_, nf), 256, 0, stream_>>>(
       sn, d_df_fwd_, d_df_fend_, nf, d_df_tgt_, arena, (const T*)d_dinv_, d_lindx0_, d_invb_, Y, X,

@@ -38,6 +38,10 @@ struct EngineOptions {
   int solve_dataflow = 0; // dependency-driven solve sweeps: disabled, requesting it is an error
   int solve_inverse = 1;  // partitioned-inverse solve where every supernode is <= kStep wide
   int refine = 0;         // 10*ordflag + altflag of the supernode reordering, 0 = none
+  int autotune = 1;       // time the Schur launch configurations of every level on the first
+                          // factorization of a plan with at least kAutotuneMinFlops Schur flops
+  int schur_tile = 0;     // 0: tuner / plan default; 64: 64 x 64 tiles everywhere; 128: 128 x 64
+                          // tiles everywhere (no tuning when set)
   std::vector<Int> user_perm;
 };
 
@@ -48,6 +52,21 @@ struct KindStats {
   int64_t launches = 0;
   double ms = 0.0;        // summed event time (profile mode)
 };
+
+// One Schur launch range as the autotuner left it: the default
+// configuration (plan order, heuristic k split) and the fastest one measured.
+struct TuneRecord {
+  int level;
+  int64_t tiles;
+  int split0, alt0;
+  double ms0;         // best time of the default configuration
+  int split, alt;     // chosen configuration
+  double ms;          // its best time
+  int candidates;     // configurations timed
+};
+// plans with fewer Schur flops are not tuned (their factorization takes
+// about as long as tuning it)
+constexpr double kAutotuneMinFlops = 1e9;
 
 struct SolveReport {
   int iterations = 0;       // refinement steps taken (0 for plain FP64)
@@ -106,6 +125,8 @@ class Engine {
     int level;  // etree level of the launch (-1: outside the level sequences)
   };
   std::vector<LaunchRecord> trace;    // per launch of the last profiled calls
+  std::vector<TuneRecord> tune_log;   // per Schur launch range, after autotuning
+  bool tuned() const { return tuned_; }
   void reset_stats();
   double last_factor_ms = 0.0, last_solve_ms = 0.0;
   SolveReport last_report;
@@ -115,6 +136,14 @@ class Engine {
   void free_device();
   template <typename T>
   void factor_sequence(T* arena);
+  template <typename T>
+  void launch_schur(const LaunchRange& r, int alt, int ks, T* arena, hipStream_t st);
+  // Times every Schur launch range under each candidate configuration (task
+  // order x k split) on the arena before the factorization assembles it, and
+  // keeps the fastest in plan_.factor_seq.
+  template <typename T>
+  void autotune_schur(T* arena);
+  bool tuned_ = false;
   template <typename T>
   void launch_diag(int bucket, bool ldl, unsigned g, const Task4* t, T* arena);
   template <typename T, bool BWD>
@@ -206,7 +235,8 @@ class Engine {
   int32_t* d_mmap_ = nullptr;
   int64_t* d_amap_ = nullptr;
   int32_t* d_invp0_ = nullptr;  // original -> permuted row, 0-based
-  void* d_tasks_[9] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused, partitioned-inverse tiles
+  void* d_tasks_[12] = {nullptr};  // diag, trsm, inpanel, schur, sdiag, soff, soffm, fused, partitioned-inverse tiles,
+                                   // schur longest first, schur 128 x 64 (plan order, longest first)
   int* d_info_ = nullptr;
   int* d_fcnt_ = nullptr;  // per supernode: loads of the fused block factor + TRSM
   // dependency-driven solve
@@ -220,6 +250,7 @@ class Engine {
   int64_t* d_invb_ = nullptr;
   int* d_df_state_ = nullptr;  // head[2], dcnt, dflag, rcnt, bflag, abort
   int df_grid_ = 0;
+  int df_nf_ = 0, df_nb_ = 0;  // task counts of the uploaded dataflow lists
   int64_t* d_acolptr_ = nullptr;
   int32_t* d_arowind_ = nullptr;
   int32_t* d_acolidx_ = nullptr;

@@ -649,28 +649,37 @@ __device__ __forceinline__ void ptrsm_compute(const SnInfo& s, int c0, int bs, i
 #pragma unroll
   for (int K = 0; K < NBM; ++K) {
     if (K < nb) {
-      // two accumulators (even / odd J) halve the dependent MFMA chain
-      v4 y2 = v4{0, 0, 0, 0};
+      // every LDS operand of block row K (the tiles L_KJ, J < K, and the
+      // inverse tile) is read before the row's first product, so the
+      // products issue back to back instead of each waiting for its read
+      T lv[NBM][4], li[4];
 #pragma unroll
       for (int J = 0; J < K; ++J) {
         const T* L = tl + tslot(K, J) * TSZ + (lane & 15);
 #pragma unroll
+        for (int kk = 0; kk < 4; ++kk) lv[J][kk] = L[Mfma<T>::row(lane, kk) * TSTR];
+      }
+      {
+        const T* Li = tl + tslot(K, K) * TSZ + (lane & 15);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) li[kk] = Li[Mfma<T>::row(lane, kk) * TSTR];
+      }
+      // two accumulators (even / odd J) halve the dependent MFMA chain
+      v4 y2 = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int J = 0; J < K; ++J) {
+#pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-          const int kp = Mfma<T>::row(lane, kk);
           if (J & 1)
-            y2 = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y2);
+            y2 = Mfma<T>::mma(-lv[J][kk], y[J][kk], y2);
           else
-            y[K] = Mfma<T>::mma(-L[kp * TSTR], y[J][kk], y[K]);
+            y[K] = Mfma<T>::mma(-lv[J][kk], y[J][kk], y[K]);
         }
       }
       y[K] += y2;
-      const T* Li = tl + tslot(K, K) * TSZ + (lane & 15);
       v4 z = v4{0, 0, 0, 0};
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const int kp = Mfma<T>::row(lane, kk);
-        z = Mfma<T>::mma(Li[kp * TSTR], y[K][kk], z);
-      }
+      for (int kk = 0; kk < 4; ++kk) z = Mfma<T>::mma(li[kk], y[K][kk], z);
       y[K] = z;
     }
   }
@@ -736,6 +745,7 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   __shared__ T As[KC][LDS_LD];
   __shared__ T Bs[KC][LDS_LD];
   __shared__ T Dk[KC];
+  __shared__ int64_t sTc[TILE], sTm[TILE];  // MODE 1: scatter bases of the tile's output columns
   // grid-stride over (task, k part) pairs (the launches use one workgroup
   // per pair); part `part` of a task covers k chunks [q_lo, q_hi)
   for (int64_t gi = blockIdx.x; gi < ntasks * ksplit; gi += gridDim.x) {
@@ -851,20 +861,36 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
   // and columns a of the lower half (wb > wa) only produces entries a < b,
   // which are never stored: it skips its products.
   const bool idle = (ti == tj) && (wb > wa);
+  // MODE 1: the scatter bases tcol / tmap of the tile's 64 output columns b
+  // are loaded with the first chunk and kept in LDS, so the epilogue needs
+  // one memory round trip (rel) instead of two, and that one is issued
+  // before the last chunk's products
+  int64_t tcv = 0, tmv = 0;
+  const int m = s.r - s.w;
+  if (MODE == 1 && tid < TILE) {
+    const int bc = min(tj * TILE + tid, m - 1);
+    tcv = tcol[s.upd + bc];
+    tmv = tmap[s.upd + bc];
+  }
   fetch(q_lo);
-  for (int qc = q_lo; qc < q_hi; ++qc) {
+  auto stage = [&](int qc) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       As[lk + 4 * q][lrow] = ((mk >> q) & 1u) ? ra[q] : T(0);
       Bs[lk + 4 * q][lrow] = ((mk >> (q + NQ)) & 1u) ? rb[q] : T(0);
     }
     if (LDL && tid < KC) Dk[tid] = ((mk >> (2 * NQ)) & 1u) ? dv : T(0);
+    if (MODE == 1 && qc == q_lo && tid < TILE) {
+      sTc[tid] = tcv;
+      sTm[tid] = tmv;
+    }
     __syncthreads();
+  };
+  auto products = [&](int qc) {
     // k steps of 4 that hold valid columns in this chunk (the last chunk of
     // each part is partial: its zero-filled tail is not multiplied)
     const int krem = qc < nq1 ? K - qc * KC : K2 - (qc - nq1) * KC;
     const int nkk = idle ? 0 : min(KC / 4, (krem + 3) >> 2);
-    if (qc + 1 < q_hi) fetch(qc + 1);  // next chunk in flight during the MFMAs
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
       if (kk >= nkk) break;  // uniform over the wave
@@ -882,40 +908,38 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
 #pragma unroll
         for (int aj = 0; aj < 2; ++aj) acc[bi][aj] = Mfma<T>::mma(bf[bi], af[aj], acc[bi][aj]);
     }
+  };
+  for (int qc = q_lo; qc < q_hi - 1; ++qc) {
+    stage(qc);
+    fetch(qc + 1);  // next chunk in flight during the MFMAs
+    products(qc);
     __syncthreads();
   }
-
-  // MODE 1: scatter offsets of the 16 outputs of this lane (rel is defined
-  // for rows a >= the first row of b's target group: clamped, unconditional
-  // loads).  Loaded after the k loop: out of the loop's registers (124
-  // VGPRs) four workgroups share a CU, which hides their latency better
-  // than loading them with the first chunk (148 VGPRs, three workgroups).
+  // last chunk (peeled: the prefetch registers are free, the scatter
+  // offsets take their place)
+  stage(q_hi - 1);
+  // MODE 1: scatter offsets of the 16 outputs of this lane, target column
+  // base from LDS plus the relative row (rel is defined for rows a >= the
+  // first row of b's target group: clamped, unconditional loads)
   int64_t off[2][4][2];
   if (MODE == 1) {
-    const int m = s.r - s.w;
-    int64_t tm[2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
-        off[bi][v][0] = tcol[s.upd + bc];
-        tm[bi][v] = tmap[s.upd + bc];
-      }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int bc = min(tj * TILE + wb * 32 + bi * 16 + Mfma<T>::row(lane, v), m - 1);
-        const int64_t tc = off[bi][v][0];
+        const int bl = wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+        const int bc = min(tj * TILE + bl, m - 1);
+        const int64_t tc = sTc[bl], tmb = sTm[bl];
 #pragma unroll
         for (int aj = 0; aj < 2; ++aj) {
           const int aoff = ti * TILE + wa * 32 + aj * 16 + (lane & 15);
           const int ac = min(max(aoff, bc), m - 1);
-          off[bi][v][aj] = tc + rel[tm[bi][v] + ac];
+          off[bi][v][aj] = tc + rel[tmb + ac];
         }
       }
   }
+  products(q_hi - 1);
+
   // Epilogue: every load first (clamped, unconditional, one wait), then the
   // predicated stores / atomics.
   if (MODE == 0) {
@@ -948,7 +972,6 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
             P[(int64_t)bg * ld + ag] = cur[bi][aj][v] - acc[bi][aj][v];
         }
   } else {
-    const int m = s.r - s.w;
     // a deferred supernode scatters only the output columns in its parent
     const int bcut = t.d < 0 ? -t.d : m;
 #pragma unroll
@@ -963,6 +986,210 @@ __global__ __launch_bounds__(NT) void k_update(const SnInfo* __restrict__ sn,
         }
       }
   }
+  // a workgroup that takes another (task, part) rewrites As / Bs / sTc only
+  // after every wave has finished reading them
+  if (gi + gridDim.x < ntasks * ksplit) __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Schur update with 128 x 64 output tiles (rows a x columns b; the
+// autotuner's wide candidate).  Same algorithm as k_update<T,1,LDL>: C =
+// -P_a P_b^T (LDL: -P_a D P_b^T) over the supernode's k columns (and a merged
+// child's, gathered through the row map), K in chunks of 32 staged through
+// LDS with the next chunk in flight, scatter through tcol/tmap/rel with
+// atomics.  Each wave owns 64 a-rows x 32 b-columns, 4 x 2 MFMA 16x16x4
+// blocks: every k step reads 6 LDS operands for 8 products (4 for 4 in the
+// 64 x 64 tile), and a tile reads 192 rows of operands for 8192 outputs
+// (128 for 4096).  Task {sn, 128-row tile ti, 64-column tile tj, d} with
+// 64 tj <= 128 ti + 127; d as in k_update.
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) void k_schur_wide(const SnInfo* __restrict__ sn,
+                                                    const Task4* __restrict__ tasks,
+                                                    T* __restrict__ arena,
+                                                    const int64_t* __restrict__ tcol,
+                                                    const int64_t* __restrict__ tmap,
+                                                    const int32_t* __restrict__ rel,
+                                                    const MergeInfo* __restrict__ merges,
+                                                    const int32_t* __restrict__ mmap, int64_t ntasks,
+                                                    int ksplit) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int KC = kSchurKC;
+  constexpr int TA = 2 * TILE, TB = TILE;
+  constexpr int NQ = KC / 4;
+  __shared__ T As[KC][TA + 1];  // odd pitches: the 4 k rows a wave reads hit different banks
+  __shared__ T Bs[KC][TB + 1];
+  __shared__ T Dk[KC];
+  __shared__ int64_t sTc[TB], sTm[TB];
+  for (int64_t gi = blockIdx.x; gi < ntasks * ksplit; gi += gridDim.x) {
+    const int64_t task = gi / ksplit;
+    const int part = (int)(gi - task * ksplit);
+    const Task4 t = tasks[task];
+    const SnInfo s = sn[t.a];
+    const T* __restrict__ P = arena + s.off;
+    const int64_t ld = s.r;
+    const int K = s.w, m = s.r - s.w;
+    const int ti = t.b, tj = t.c;
+    const int arow0 = s.w + ti * TA, brow0 = s.w + tj * TB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wb = wave >> 1, wa = wave & 1;
+    v4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
+    // loader: a wave reads 64 consecutive rows of one k column; A rows in two
+    // halves of 64
+    const int lrow = tid & 63, lk = tid >> 6;
+    bool aok[2];
+    int arow[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      aok[h] = arow0 + h * 64 + lrow < s.r;
+      arow[h] = min(arow0 + h * 64 + lrow, s.r - 1);
+    }
+    const bool bok = brow0 + lrow < s.r;
+    const int brow = min(brow0 + lrow, s.r - 1);
+    int K2 = 0;
+    const T* __restrict__ P2 = P;
+    int64_t ld2 = 1;
+    int arow2[2] = {0, 0}, brow2 = 0;
+    bool aok2[2] = {false, false}, bok2 = false;
+    if (t.d > 0) {
+      const MergeInfo mg = merges[t.d - 1];
+      K2 = mg.w;
+      P2 = arena + mg.off;
+      ld2 = mg.ld;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int ma = mmap[mg.moff + (arow[h] - s.w)];
+        aok2[h] = aok[h] && ma >= 0;
+        arow2[h] = max(ma, 0);
+      }
+      const int mb = mmap[mg.moff + (brow - s.w)];
+      bok2 = bok && mb >= 0;
+      brow2 = max(mb, 0);
+    }
+    const int nq1 = (K + KC - 1) / KC;
+    const int nq = nq1 + (K2 + KC - 1) / KC;
+    const int q_lo = (int)((int64_t)part * nq / ksplit), q_hi = (int)((int64_t)(part + 1) * nq / ksplit);
+    if (q_lo >= q_hi) continue;  // uniform over the workgroup
+    // chunk registers: A halves, B; validity bits applied at the LDS write
+    T ra[2][NQ], rb[NQ], dv = T(0);
+    unsigned mk = 0;
+    auto fetch = [&](int qc) {
+      unsigned mm = 0;
+      const bool own = qc < nq1;
+      const int kc = (own ? qc : qc - nq1) * KC;
+      const int Kp = own ? K : K2;
+      const T* __restrict__ Pp = own ? P : P2;
+      const int64_t ldp = own ? ld : ld2;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int kg = kc + lk + 4 * q;
+        const T* col = Pp + (int64_t)min(kg, Kp - 1) * ldp;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          ra[h][q] = col[own ? arow[h] : arow2[h]];
+          mm |= (unsigned)((kg < Kp) & (own ? aok[h] : aok2[h])) << (h * NQ + q);
+        }
+        rb[q] = col[own ? brow : brow2];
+        mm |= (unsigned)((kg < Kp) & (own ? bok : bok2)) << (2 * NQ + q);
+      }
+      if (LDL) {
+        const int kg = min(kc + (tid & (KC - 1)), Kp - 1);
+        dv = Pp[(int64_t)kg * ldp + kg];
+        mm |= (unsigned)(kc + (tid & (KC - 1)) < Kp) << (3 * NQ);
+      }
+      mk = mm;
+    };
+    // a wave whose 64 x 32 block lies wholly above the diagonal (every a < b)
+    // skips its products
+    const bool idle = ti * TA + wa * 64 + 63 < tj * TB + wb * 32;
+    int64_t tcv = 0, tmv = 0;
+    if (tid < TB) {
+      const int bc = min(tj * TB + tid, m - 1);
+      tcv = tcol[s.upd + bc];
+      tmv = tmap[s.upd + bc];
+    }
+    fetch(q_lo);
+    auto stage = [&](int qc) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) As[lk + 4 * q][h * 64 + lrow] = ((mk >> (h * NQ + q)) & 1u) ? ra[h][q] : T(0);
+        Bs[lk + 4 * q][lrow] = ((mk >> (2 * NQ + q)) & 1u) ? rb[q] : T(0);
+      }
+      if (LDL && tid < KC) Dk[tid] = ((mk >> (3 * NQ)) & 1u) ? dv : T(0);
+      if (qc == q_lo && tid < TB) {
+        sTc[tid] = tcv;
+        sTm[tid] = tmv;
+      }
+      __syncthreads();
+    };
+    auto products = [&](int qc) {
+      const int krem = qc < nq1 ? K - qc * KC : K2 - (qc - nq1) * KC;
+      const int nkk = idle ? 0 : min(KC / 4, (krem + 3) >> 2);
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; ++kk) {
+        if (kk >= nkk) break;  // uniform over the wave
+        const int k = kk * 4 + (lane >> 4);
+        T bf[2], af[4];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          bf[bi] = Bs[k][wb * 32 + bi * 16 + (lane & 15)];
+          if (LDL) bf[bi] *= Dk[k];
+        }
+#pragma unroll
+        for (int aj = 0; aj < 4; ++aj) af[aj] = As[k][wa * 64 + aj * 16 + (lane & 15)];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int aj = 0; aj < 4; ++aj) acc[bi][aj] = Mfma<T>::mma(bf[bi], af[aj], acc[bi][aj]);
+      }
+    };
+    for (int qc = q_lo; qc < q_hi - 1; ++qc) {
+      stage(qc);
+      fetch(qc + 1);
+      products(qc);
+      __syncthreads();
+    }
+    stage(q_hi - 1);
+    // scatter offsets: column base from LDS, relative row per output (rel is
+    // defined for rows a >= the first row of b's target group: clamped,
+    // unconditional loads), issued before the last chunk's products
+    int64_t tcb[2][4];
+    int32_t relv[2][4][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int bl = wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+        const int bc = min(tj * TB + bl, m - 1);
+        tcb[bi][v] = sTc[bl];
+        const int64_t tmb = sTm[bl];
+#pragma unroll
+        for (int aj = 0; aj < 4; ++aj) {
+          const int aoff = ti * TA + wa * 64 + aj * 16 + (lane & 15);
+          relv[bi][v][aj] = rel[tmb + min(max(aoff, bc), m - 1)];
+        }
+      }
+    products(q_hi - 1);
+    const int bcut = t.d < 0 ? -t.d : m;  // deferred supernode: its parent's columns only
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int boff = tj * TB + wb * 32 + bi * 16 + Mfma<T>::row(lane, v);
+#pragma unroll
+        for (int aj = 0; aj < 4; ++aj) {
+          const int aoff = ti * TA + wa * 64 + aj * 16 + (lane & 15);
+          if (boff < bcut && aoff < m && aoff >= boff)
+            atomic_add(arena + tcb[bi][v] + relv[bi][v][aj], -acc[bi][aj][v]);
+        }
+      }
+    if (gi + gridDim.x < ntasks * ksplit) __syncthreads();
   }
 }
 
@@ -1522,29 +1749,37 @@ __global__ __launch_bounds__(NT) void k_gprep(const SnInfo* __restrict__ sn, con
   for (int Ks = 0; Ks < NBM; ++Ks) {
     const int K = NBM - 1 - Ks;
     if (K < nb) {
+      // the LDS operands of step K (L_JK^T for J > K, inv(L_KK)^T) are read
+      // before its first product (clamped reads for J >= nb: not used)
+      T lv[NBM][4], li[4];
+#pragma unroll
+      for (int J = K + 1; J < NBM; ++J) {
+        const T* L = tl + tslot(min(J, nb - 1), K) * TSZ + (lane & 15) * TSTR;  // L_JK^T[m][k] = L_JK[k][m]
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) lv[J][kk] = L[Mfma<T>::row(lane, kk)];
+      }
+      {
+        const T* Li = tl + tslot(K, K) * TSZ + (lane & 15) * TSTR;  // inv(L_KK)^T
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) li[kk] = Li[Mfma<T>::row(lane, kk)];
+      }
       v4 y2 = v4{0, 0, 0, 0};
 #pragma unroll
-      for (int J = 0; J < NBM; ++J) {
-        if (J > K && J < nb) {
-          const T* L = tl + tslot(J, K) * TSZ + (lane & 15) * TSTR;  // L_JK^T[m][k] = L_JK[k][m]
+      for (int J = K + 1; J < NBM; ++J) {
+        if (J < nb) {
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
-            const int kp = Mfma<T>::row(lane, kk);
             if (J & 1)
-              y2 = Mfma<T>::mma(-L[kp], y[J][kk], y2);
+              y2 = Mfma<T>::mma(-lv[J][kk], y[J][kk], y2);
             else
-              y[K] = Mfma<T>::mma(-L[kp], y[J][kk], y[K]);
+              y[K] = Mfma<T>::mma(-lv[J][kk], y[J][kk], y[K]);
           }
         }
       }
       y[K] += y2;
-      const T* Li = tl + tslot(K, K) * TSZ + (lane & 15) * TSTR;  // inv(L_KK)^T
       v4 z = v4{0, 0, 0, 0};
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const int kp = Mfma<T>::row(lane, kk);
-        z = Mfma<T>::mma(Li[kp], y[K][kk], z);
-      }
+      for (int kk = 0; kk < 4; ++kk) z = Mfma<T>::mma(li[kk], y[K][kk], z);
       y[K] = z;
     }
   }

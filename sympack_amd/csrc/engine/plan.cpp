@@ -321,6 +321,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       if (ur.count) p.factor_seq.push_back(ur);
     }
     LaunchRange sr{2, (int64_t)p.schur_tasks.size(), 0};
+    sr.first_w = (int64_t)p.schur_wide.size();
     for (Int I : sns) {
       const SnInfo& si = p.sn[I];
       int64_t m = si.r - si.w;
@@ -334,11 +335,17 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
         for (int64_t ti = 0; ti < nt; ++ti)
           for (int64_t tj = 0; tj <= ti && tj < ntj; ++tj)
             p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, -n1});
+        for (int64_t ti = 0; ti < ceil_div(m, 2 * kTile); ++ti)
+          for (int64_t tj = 0; tj <= 2 * ti + 1 && tj < ntj; ++tj)
+            p.schur_wide.push_back({I, (int32_t)ti, (int32_t)tj, -n1});
         p.schur_entries += (int64_t)n1 * m - (int64_t)n1 * (n1 - 1) / 2;
       } else {
         for (int64_t ti = 0; ti < nt; ++ti)
           for (int64_t tj = 0; tj <= ti; ++tj)
             p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, merge_of[I]});
+        for (int64_t ti = 0; ti < ceil_div(m, 2 * kTile); ++ti)
+          for (int64_t tj = 0; tj <= 2 * ti + 1 && tj < nt; ++tj)
+            p.schur_wide.push_back({I, (int32_t)ti, (int32_t)tj, merge_of[I]});
         p.schur_entries += m * (m + 1) / 2;
       }
       // algorithmic flops stay those of the supernode's whole update, wherever
@@ -346,6 +353,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       p.schur_flops += (double)si.w * (double)m * (double)(m + 1);
     }
     sr.count = (int64_t)p.schur_tasks.size() - sr.first;
+    sr.count_w = (int64_t)p.schur_wide.size() - sr.first_w;
     if (sr.count && sr.count < kSplitTarget) {
       // few output tiles: each task's k chunks are split over S workgroups
       // (partial sums meet in the atomic scatter), S <= the fewest chunks of
@@ -361,6 +369,32 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
     if (sr.count) p.factor_seq.push_back(sr);
     for (size_t q = level_first; q < p.factor_seq.size(); ++q) p.factor_seq[q].level = L;
+  }
+  // longest-first copy of every Schur range (k chunks of the panel and of a
+  // merged child)
+  // (a stable counting sort on the chunk count, a small integer)
+  p.schur_tasks_lpt.resize(p.schur_tasks.size());
+  p.schur_wide_lpt.resize(p.schur_wide.size());
+  {
+    auto chunks = [&](const Task4& t) {
+      int nq = (int)ceil_div(p.sn[t.a].w, kSchurKC);
+      if (t.d > 0) nq += (int)ceil_div(p.merges[t.d - 1].w, kSchurKC);
+      return nq;
+    };
+    std::vector<int64_t> cnt;
+    auto longest_first = [&](const std::vector<Task4>& src, std::vector<Task4>& dst, int64_t first, int64_t count) {
+      int maxq = 0;
+      for (int64_t q = first; q < first + count; ++q) maxq = std::max(maxq, chunks(src[q]));
+      cnt.assign(maxq + 2, 0);
+      for (int64_t q = first; q < first + count; ++q) ++cnt[maxq - chunks(src[q]) + 1];
+      for (int i = 1; i <= maxq + 1; ++i) cnt[i] += cnt[i - 1];
+      for (int64_t q = first; q < first + count; ++q) dst[first + cnt[maxq - chunks(src[q])]++] = src[q];
+    };
+    for (const LaunchRange& r : p.factor_seq) {
+      if (r.kind != 2) continue;
+      longest_first(p.schur_tasks, p.schur_tasks_lpt, r.first, r.count);
+      longest_first(p.schur_wide, p.schur_wide_lpt, r.first_w, r.count_w);
+    }
   }
 
   // ---- solve task lists ----
@@ -416,49 +450,8 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     }
   }
 
-  // dependency-driven solve task lists
+  // the dependency-driven solve's lists are built on demand (build_dataflow_lists)
   p.df_ok = p.max_w <= kStep;
-  if (p.df_ok) {
-    p.df_dcnt0.assign(ns, 0);
-    p.df_rcnt0.assign(ns, 0);
-    std::vector<int32_t> last_seen(ns, -1);
-    std::vector<int64_t> dtask(ns, 0);  // forward D task of every supernode
-    int32_t stamp = 0;
-    for (Int I = 0; I < ns; ++I) {
-      const SnInfo& si = p.sn[I];
-      const int32_t m = si.r - si.w;
-      const int32_t nt = (int32_t)ceil_div(m, kTile);
-      p.df_rcnt0[I] = nt;
-      dtask[I] = (int64_t)p.df_fwd.size();
-      p.df_fwd.push_back({0, I, 0, 0});
-      p.df_fwd_end.push_back(0);
-      for (int32_t rt = 0; rt < nt; ++rt) {
-        const int32_t first = (int32_t)p.df_tgt.size();
-        ++stamp;
-        for (int32_t a = rt * kTile; a < std::min<int32_t>(m, (rt + 1) * kTile); ++a) {
-          const int32_t T = row_sup[p.lindx0[si.lx + si.w + a]];
-          if (last_seen[T] != stamp) {
-            last_seen[T] = stamp;
-            p.df_tgt.push_back(T);
-            ++p.df_dcnt0[T];
-          }
-        }
-        p.df_fwd.push_back({1, I, rt, first});
-        p.df_fwd_end.push_back((int32_t)p.df_tgt.size());
-      }
-    }
-    // backward: the same tiles in reverse postorder, waiting for the owners
-    // of their rows
-    for (Int I = ns - 1; I >= 0; --I) {
-      for (int32_t rt = 0; rt < p.df_rcnt0[I]; ++rt) {
-        const int64_t q = dtask[I] + 1 + rt;
-        p.df_bwd.push_back({2, I, rt, p.df_fwd[q].d});
-        p.df_bwd_end.push_back(p.df_fwd_end[q]);
-      }
-      p.df_bwd.push_back({3, I, 0, 0});
-      p.df_bwd_end.push_back(0);
-    }
-  }
 
   // forward: levels bottom-up, diagonal solves then off-diagonal updates.
   std::vector<int> dbucket(p.nlevels, 1);
@@ -484,10 +477,62 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   }
 }
 
+
+DataflowLists build_dataflow_lists(const Plan& p) {
+  DataflowLists L;
+  if (!p.df_ok) return L;
+  const Int ns = p.nsuper;
+  std::vector<int32_t> row_sup(p.n);
+  for (Int I = 0; I < ns; ++I)
+    for (int32_t c = p.sn[I].fc; c < p.sn[I].fc + p.sn[I].w; ++c) row_sup[c] = I;
+
+    L.dcnt0.assign(ns, 0);
+    L.rcnt0.assign(ns, 0);
+    std::vector<int32_t> last_seen(ns, -1);
+    std::vector<int64_t> dtask(ns, 0);  // forward D task of every supernode
+    int32_t stamp = 0;
+    for (Int I = 0; I < ns; ++I) {
+      const SnInfo& si = p.sn[I];
+      const int32_t m = si.r - si.w;
+      const int32_t nt = (int32_t)ceil_div(m, kTile);
+      L.rcnt0[I] = nt;
+      dtask[I] = (int64_t)L.fwd.size();
+      L.fwd.push_back({0, I, 0, 0});
+      L.fwd_end.push_back(0);
+      for (int32_t rt = 0; rt < nt; ++rt) {
+        const int32_t first = (int32_t)L.tgt.size();
+        ++stamp;
+        for (int32_t a = rt * kTile; a < std::min<int32_t>(m, (rt + 1) * kTile); ++a) {
+          const int32_t T = row_sup[p.lindx0[si.lx + si.w + a]];
+          if (last_seen[T] != stamp) {
+            last_seen[T] = stamp;
+            L.tgt.push_back(T);
+            ++L.dcnt0[T];
+          }
+        }
+        L.fwd.push_back({1, I, rt, first});
+        L.fwd_end.push_back((int32_t)L.tgt.size());
+      }
+    }
+    // backward: the same tiles in reverse postorder, waiting for the owners
+    // of their rows
+    for (Int I = ns - 1; I >= 0; --I) {
+      for (int32_t rt = 0; rt < L.rcnt0[I]; ++rt) {
+        const int64_t q = dtask[I] + 1 + rt;
+        L.bwd.push_back({2, I, rt, L.fwd[q].d});
+        L.bwd_end.push_back(L.fwd_end[q]);
+      }
+      L.bwd.push_back({3, I, 0, 0});
+      L.bwd_end.push_back(0);
+    }
+  return L;
+}
+
 std::string validate_solve_plan(const Plan& p) {
   if (!p.df_ok) return std::string();
+  const DataflowLists L = build_dataflow_lists(p);
   const Int ns = p.nsuper;
-  std::vector<int32_t> dcnt = p.df_dcnt0, rcnt = p.df_rcnt0;
+  std::vector<int32_t> dcnt = L.dcnt0, rcnt = L.rcnt0;
   std::vector<char> dflag(ns, 0), bflag(ns, 0);
   std::vector<int32_t> row_sup(p.n, -1);
   for (Int I = 0; I < ns; ++I)
@@ -500,32 +545,32 @@ std::string validate_solve_plan(const Plan& p) {
       const int32_t T = row_sup[p.lindx0[si.lx + si.w + a]];
       if (std::find(want.begin(), want.end(), T) == want.end()) want.push_back(T);
     }
-    std::vector<int32_t> got(p.df_tgt.begin() + t.d, p.df_tgt.begin() + end);
+    std::vector<int32_t> got(L.tgt.begin() + t.d, L.tgt.begin() + end);
     std::sort(want.begin(), want.end());
     std::sort(got.begin(), got.end());
     return want == got;
   };
-  for (size_t q = 0; q < p.df_fwd.size(); ++q) {
-    const Task4& t = p.df_fwd[q];
+  for (size_t q = 0; q < L.fwd.size(); ++q) {
+    const Task4& t = L.fwd[q];
     if (t.a == 0) {
       if (dcnt[t.b] != 0) return "forward D(" + std::to_string(t.b) + ") reached with pending updates";
       dflag[t.b] = 1;
     } else if (t.a == 1) {
       if (!dflag[t.b]) return "forward O(" + std::to_string(t.b) + ") before its diagonal solve";
-      if (!owners_ok(t, p.df_fwd_end[q])) return "forward O(" + std::to_string(t.b) + ") targets";
-      for (int32_t k = t.d; k < p.df_fwd_end[q]; ++k) --dcnt[p.df_tgt[k]];
+      if (!owners_ok(t, L.fwd_end[q])) return "forward O(" + std::to_string(t.b) + ") targets";
+      for (int32_t k = t.d; k < L.fwd_end[q]; ++k) --dcnt[L.tgt[k]];
     } else {
       return "bad forward task kind";
     }
   }
   for (Int I = 0; I < ns; ++I)
     if (dcnt[I] != 0 || !dflag[I]) return "forward counters not drained";
-  for (size_t q = 0; q < p.df_bwd.size(); ++q) {
-    const Task4& t = p.df_bwd[q];
+  for (size_t q = 0; q < L.bwd.size(); ++q) {
+    const Task4& t = L.bwd[q];
     if (t.a == 2) {
-      for (int32_t k = t.d; k < p.df_bwd_end[q]; ++k)
-        if (!bflag[p.df_tgt[k]]) return "backward R(" + std::to_string(t.b) + ") before an owner is solved";
-      if (!owners_ok(t, p.df_bwd_end[q])) return "backward R(" + std::to_string(t.b) + ") targets";
+      for (int32_t k = t.d; k < L.bwd_end[q]; ++k)
+        if (!bflag[L.tgt[k]]) return "backward R(" + std::to_string(t.b) + ") before an owner is solved";
+      if (!owners_ok(t, L.bwd_end[q])) return "backward R(" + std::to_string(t.b) + ") targets";
       --rcnt[t.b];
     } else if (t.a == 3) {
       if (rcnt[t.b] != 0) return "backward B(" + std::to_string(t.b) + ") reached with pending tiles";

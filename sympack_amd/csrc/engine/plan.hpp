@@ -84,6 +84,9 @@ struct LaunchRange {
   int aux = 0;       // kinds 0, 7, 9, 3, 6: diagonal-block bucket (diag_bucket);
                      // kind 2: k-chunk split factor (0 or 1 = none)
   int level = 0;     // etree level of the launch (factor sequence)
+  int alt = 0;       // kind 2: bit 0 = tasks in longest-first order (schur_tasks_lpt /
+                     // schur_wide_lpt), bit 1 = 128 x 64 tiles (k_schur_wide)
+  int64_t first_w = 0, count_w = 0;  // kind 2: the level's 128 x 64 tile tasks
 };
 
 struct Plan {
@@ -114,6 +117,13 @@ struct Plan {
   // carries merges[d-1] (a deferred child), d < 0 only output columns b < -d
   // (parent columns of a deferred supernode)
   std::vector<Task4> schur_tasks;
+  // the same tasks with every launch range's tasks ordered by their k
+  // chunks, longest first (stable): an alternative task order for the
+  // autotuner (workgroups are dispatched in index order)
+  std::vector<Task4> schur_tasks_lpt;
+  // the same updates as 128 x 64 output tiles {sn, 128-row tile, 64-column
+  // tile, d} (64 tj <= 128 ti + 127), plain and longest-first per range
+  std::vector<Task4> schur_wide, schur_wide_lpt;
   std::vector<MergeInfo> merges;
   std::vector<int32_t> mmap;
   std::vector<int32_t> deferred_into;  // per supernode: parent absorbing its update, -1 = none
@@ -138,12 +148,7 @@ struct Plan {
   // postorder, backward R(s,rt) = 2 / B(s) = 3 in reverse postorder; the
   // targets of O(s,rt) / R(s,rt), df_tgt[d .. end), are the supernodes
   // owning the tile's rows.
-  bool df_ok = false;
-  std::vector<Task4> df_fwd, df_bwd;
-  std::vector<int32_t> df_fwd_end, df_bwd_end;
-  std::vector<int32_t> df_tgt;          // shared by O and R tasks (same tiles)
-  std::vector<int32_t> df_dcnt0;        // per supernode: O tasks that update its rows
-  std::vector<int32_t> df_rcnt0;        // per supernode: its row tiles
+  bool df_ok = false;  // the lists can be built (build_dataflow_lists)
   // algorithmic counts
   double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
   int64_t schur_entries = 0;       // lower-triangle entries scattered
@@ -156,6 +161,17 @@ struct Plan {
 // 2 merge every qualifying chain child.
 void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, int baseval,
                 Plan& p, int merge_updates = 1);
+
+// Task lists of the dependency-driven solve (switched off on the device):
+// built on demand from the plan, not by build_plan.
+struct DataflowLists {
+  std::vector<Task4> fwd, bwd;
+  std::vector<int32_t> fwd_end, bwd_end;
+  std::vector<int32_t> tgt;          // shared by O and R tasks (same tiles)
+  std::vector<int32_t> dcnt0;        // per supernode: O tasks that update its rows
+  std::vector<int32_t> rcnt0;        // per supernode: its row tiles
+};
+DataflowLists build_dataflow_lists(const Plan& p);
 
 // Checks the dependency-driven solve lists by running them in ticket order
 // on one worker: every wait must already be satisfied when its task is

@@ -73,7 +73,7 @@ class symPACKOptions:
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
                  profile=False, device=0, merge_updates=1, solve_dataflow=False,
-                 order_refinement_str="NO", solve_inverse=True):
+                 order_refinement_str="NO", solve_inverse=True, autotune=True, schur_tile=0):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -92,6 +92,11 @@ class symPACKOptions:
         self.order_refinement_str = order_refinement_str
         # partitioned-inverse solve where every supernode is <= 160 wide
         self.solve_inverse = solve_inverse
+        # time the Schur launch configurations per etree level on the first
+        # factorization (structures with >= 1e9 Schur flops)
+        self.autotune = autotune
+        # Schur output tile shape: 0 autotuner / plan, 64 (64 x 64), 128 (128 x 64)
+        self.schur_tile = schur_tile
 
     @classmethod
     def from_args(cls, args):
@@ -109,7 +114,8 @@ class symPACKOptions:
                    ir_max_iter=o.ir_max_iter, ir_tol=o.ir_tol, verbose=o.verbose,
                    merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow),
                    order_refinement_str=refine_name(o.refine),
-                   solve_inverse=bool(o.solve_inverse))
+                   solve_inverse=bool(o.solve_inverse), autotune=bool(o.autotune),
+                   schur_tile=int(o.schur_tile))
 
     def to_c(self):
         lib = _lib.load()
@@ -133,6 +139,8 @@ class symPACKOptions:
         o.merge_updates = int(self.merge_updates)
         o.solve_dataflow = 1 if self.solve_dataflow else 0
         o.solve_inverse = 1 if self.solve_inverse else 0
+        o.autotune = 1 if self.autotune else 0
+        o.schur_tile = int(self.schur_tile)
         o.refine = refine_code(self.order_refinement_str)
         return o
 
@@ -327,6 +335,18 @@ class symPACKMatrix2D:
                                                         _ptr(g, ctypes.c_int64),
                                                         _ptr(ms, ctypes.c_double), max_records), "trace")
         return k[:cnt], g[:cnt], ms[:cnt]
+
+    def autotune_log(self, max_records=4096):
+        """The autotuner's record per Schur launch: dicts with level, split,
+        order (bit 0: longest first instead of plan order, bit 1: 128 x 64
+        output tiles instead of 64 x 64), ms_default and ms_best."""
+        lv, sp_, od = (np.zeros(max_records, np.int32) for _ in range(3))
+        m0, m1 = np.zeros(max_records), np.zeros(max_records)
+        cnt = _lib.check(self._lib.spk_get_autotune(self._h, _ptr(lv, ctypes.c_int32), _ptr(sp_, ctypes.c_int32),
+                                                   _ptr(od, ctypes.c_int32), _ptr(m0, ctypes.c_double),
+                                                   _ptr(m1, ctypes.c_double), max_records), "autotune")
+        return [{"level": int(lv[i]), "split": int(sp_[i]), "order": int(od[i]), "ms_default": float(m0[i]),
+                 "ms_best": float(m1[i])} for i in range(cnt)]
 
     def launch_levels(self, max_records=100000):
         """Etree level of every record of launch_trace() (-1 outside the level sequences)."""

@@ -184,3 +184,13 @@ def test_matrix_text_full_symmetric_is_reduced_to_lower(tmp_path):
 def test_unknown_matrix_format():
     with pytest.raises(ValueError, match="Unknown matrix format"):
         sp.DistSparseMatrix.read("/nonexistent", "MATRIX_MARKET")
+
+
+def test_schur_tile_option_is_validated():
+    m = sp.symPACKMatrix2D()
+    for ok in (0, 64, 128):
+        m.Init(sp.symPACKOptions(schur_tile=ok))
+    with pytest.raises(ValueError, match="schur_tile"):
+        m.Init(sp.symPACKOptions(schur_tile=96))
+    o = sp.symPACKOptions.from_args(["-schur_tile", "128", "-autotune", "0"])
+    assert o.schur_tile == 128 and o.autotune is False
