@@ -337,8 +337,9 @@ void Engine::init(const EngineOptions& o) {
 int64_t Engine::device_bytes(int nrhs) const {
   auto vb = [](const auto& v) { return (int64_t)(v.size() * sizeof(v[0])); };
   const Plan& p = plan_;
-  int64_t b = vb(p.sn) + vb(p.lindx0) + vb(p.tcol) + vb(p.tmap) + p.rel_size * 4 + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
-              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) + 2 * vb(p.schur_tasks) + 2 * vb(p.schur_wide) + vb(p.fused_tasks) +
+  int64_t b = vb(p.sn) + vb(p.lindx0) + p.upd_size * 16 + p.rel_size * 4 + vb(p.merges) + vb(p.mmap) + vb(p.amap) + vb(p.perm0) +
+              vb(p.potrf_tasks) + vb(p.trsm_tasks) + vb(p.inpanel_tasks) +
+              2 * (p.n_schur + p.n_schur_wide) * (int64_t)sizeof(Task4) + vb(p.fused_tasks) +
               vb(p.sdiag_tasks) + vb(p.soffm_tasks);
   b += (int64_t)(n_ + 1) * 8 + nnzA_ * (4 + 4 + 8) + (int64_t)n_ * 8;  // CSC, values, scaling
   b += (int64_t)n_ * 4 + 4;                                            // invp0, pivot flag
@@ -397,18 +398,19 @@ void Engine::ensure_device() {
     for (int64_t p = h_colptr_[c]; p < h_colptr_[c + 1]; ++p) colidx[p] = c;
   d_sn_ = dupload(plan_.sn, s);
   d_lindx0_ = dupload(plan_.lindx0, s);
-  d_tcol_ = dupload(plan_.tcol, s);
-  d_tmap_ = dupload(plan_.tmap, s);
   {
-    // relative indices of the Schur scatter, built on the device
+    // scatter bases and relative indices of the Schur scatter, built on the
+    // device
     RelGroup* groups = dupload(plan_.rel_groups, s);
+    d_tcol_ = dalloc<int64_t>((size_t)std::max<int64_t>(1, plan_.upd_size));
+    d_tmap_ = dalloc<int64_t>((size_t)std::max<int64_t>(1, plan_.upd_size));
     d_rel_ = dalloc<int32_t>((size_t)std::max<int64_t>(1, plan_.rel_size));
     int* bad = dalloc<int>(1);
     HIP_OK(hipMemsetAsync(bad, 0, sizeof(int), s));
     const int64_t ng = (int64_t)plan_.rel_groups.size();
     if (ng > 0) {
       dev::k_build_rel<<<(unsigned)std::min<int64_t>(ng, 65536), 256, 0, s>>>(
-          (const SnInfo*)d_sn_, d_lindx0_, groups, ng, d_rel_, bad);
+          (const SnInfo*)d_sn_, d_lindx0_, groups, ng, d_rel_, d_tcol_, d_tmap_, bad);
       HIP_OK(hipGetLastError());
     }
     int hbad = 0;
@@ -431,10 +433,23 @@ void Engine::ensure_device() {
   d_tasks_[TA_POTRF] = dupload(plan_.potrf_tasks, s);
   d_tasks_[TA_TRSM] = dupload(plan_.trsm_tasks, s);
   d_tasks_[TA_INPANEL] = dupload(plan_.inpanel_tasks, s);
-  d_tasks_[TA_SCHUR] = dupload(plan_.schur_tasks, s);
-  d_tasks_[TA_SCHUR_LPT] = dupload(plan_.schur_tasks_lpt, s);
-  d_tasks_[TA_WIDE] = dupload(plan_.schur_wide, s);
-  d_tasks_[TA_WIDE_LPT] = dupload(plan_.schur_wide_lpt, s);
+  {
+    // Schur task lists, written on the device from one record per supernode
+    for (int k : {TA_SCHUR, TA_SCHUR_LPT})
+      d_tasks_[k] = dalloc<Task4>((size_t)std::max<int64_t>(1, plan_.n_schur));
+    for (int k : {TA_WIDE, TA_WIDE_LPT})
+      d_tasks_[k] = dalloc<Task4>((size_t)std::max<int64_t>(1, plan_.n_schur_wide));
+    const int64_t ne = (int64_t)plan_.schur_emit.size();
+    if (ne > 0) {
+      SchurEmit* emit = dupload(plan_.schur_emit, s);
+      dev::k_emit_schur<<<(unsigned)std::min<int64_t>(ne, 65536), 256, 0, s>>>(
+          emit, ne, (Task4*)d_tasks_[TA_SCHUR], (Task4*)d_tasks_[TA_SCHUR_LPT], (Task4*)d_tasks_[TA_WIDE],
+          (Task4*)d_tasks_[TA_WIDE_LPT]);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(s));
+      dfree(emit);
+    }
+  }
   d_tasks_[TA_FUSED] = dupload(plan_.fused_tasks, s);
   d_tasks_[TA_SDIAG] = dupload(plan_.sdiag_tasks, s);
   d_tasks_[TA_SOFFM] = dupload(plan_.soffm_tasks, s);
@@ -921,13 +936,7 @@ void Engine::autotune_schur(T* arena) {
   bool warm = false;
   for (LaunchRange& r : plan_.factor_seq) {
     if (r.kind != 2 || r.count == 0) continue;
-    int nqmin = INT_MAX;
-    for (int64_t q = r.first; q < r.first + r.count; ++q) {
-      const Task4& t = plan_.schur_tasks[q];
-      int nq = (plan_.sn[t.a].w + kSchurKC - 1) / kSchurKC;
-      if (t.d > 0) nq += (plan_.merges[t.d - 1].w + kSchurKC - 1) / kSchurKC;
-      nqmin = std::min(nqmin, nq);
-    }
+    const int nqmin = r.nqmin;
     if (!warm) {  // first launches of the kernels in this process: load them untimed
       launch(r, 0, std::max(1, r.aux));
       if (r.count_w > 0) launch(r, 2, 1);

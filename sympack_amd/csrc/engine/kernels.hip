@@ -121,6 +121,8 @@ __global__ __launch_bounds__(256) void k_build_rel(const SnInfo* __restrict__ sn
                                                    const int32_t* __restrict__ lindx0,
                                                    const RelGroup* __restrict__ groups,
                                                    int64_t ngroups, int32_t* __restrict__ rel,
+                                                   int64_t* __restrict__ tcol,
+                                                   int64_t* __restrict__ tmap,
                                                    int* __restrict__ bad) {
   for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
     const RelGroup g = groups[gi];
@@ -128,6 +130,12 @@ __global__ __launch_bounds__(256) void k_build_rel(const SnInfo* __restrict__ sn
     const int m = si.r - si.w;
     const int32_t* __restrict__ rows = lindx0 + si.lx + si.w;
     const int32_t* __restrict__ sJ = lindx0 + sj.lx;
+    // scatter bases of the group's rows b (columns of J): the arena offset
+    // of target column b and the rel index of row a = 0
+    for (int b = g.gb + (int)threadIdx.x; b < g.ge; b += (int)blockDim.x) {
+      tcol[si.upd + b] = sj.off + (int64_t)(rows[b] - sj.fc) * sj.r;
+      tmap[si.upd + b] = g.base - g.gb;
+    }
     for (int a = g.gb + (int)threadIdx.x; a < m; a += (int)blockDim.x) {
       const int32_t row = rows[a];
       int lo = 0, hi = sj.r;  // first position whose row is >= row
@@ -141,6 +149,35 @@ __global__ __launch_bounds__(256) void k_build_rel(const SnInfo* __restrict__ sn
       const bool found = lo < sj.r && sJ[lo] == row;
       if (!found) atomicOr(bad, 1);
       rel[g.base + (a - g.gb)] = found ? lo : 0;
+    }
+  }
+}
+
+// Schur task lists from the per-supernode records: one workgroup per
+// supernode writes its 64 x 64 tiles (plan order and longest-first
+// positions) and its 128 x 64 tiles; a thread per row tile.
+__global__ __launch_bounds__(256) void k_emit_schur(const SchurEmit* __restrict__ emit, int64_t nemit,
+                                                    Task4* __restrict__ t0, Task4* __restrict__ t1,
+                                                    Task4* __restrict__ t2, Task4* __restrict__ t3) {
+  for (int64_t q = blockIdx.x; q < nemit; q += gridDim.x) {
+    const SchurEmit e = emit[q];
+    for (int ti = (int)threadIdx.x; ti < e.nt; ti += (int)blockDim.x) {
+      const int64_t at = schur_tiles_before(ti, e.ntj);
+      const int ntj = min(ti + 1, e.ntj);
+      for (int tj = 0; tj < ntj; ++tj) {
+        const Task4 t{e.I, ti, tj, e.d};
+        t0[e.first[0] + at + tj] = t;
+        t1[e.first[1] + at + tj] = t;
+      }
+    }
+    for (int ti = (int)threadIdx.x; ti < e.ntw; ti += (int)blockDim.x) {
+      const int64_t at = schur_wide_before(ti, e.ntj);
+      const int ntj = min(2 * ti + 2, e.ntj);
+      for (int tj = 0; tj < ntj; ++tj) {
+        const Task4 t{e.I, ti, tj, e.d};
+        t2[e.first[2] + at + tj] = t;
+        t3[e.first[3] + at + tj] = t;
+      }
     }
   }
 }

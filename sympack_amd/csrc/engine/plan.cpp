@@ -55,21 +55,75 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   for (Int i = 0; i < n; ++i) row_sup[i] = s.supMembership[i] - 1;
 
   // Assembly map: caller entry (row,col) of the lower triangle goes to
-  // (max,min) of the permuted indices; later duplicates win.  Entries are
-  // bucketed by their permuted column (stable, in entry order); each
-  // supernode then marks the positions of its structure rows once, so every
-  // entry finds its row in O(1).  Two entries land on the same arena offset
-  // exactly when they share the permuted pair, i.e. the bucket and the row.
-  // It runs on its own thread beside the scatter maps and the merges (it
-  // writes only amap and reads the structure).
+  // (max,min) of the permuted indices; later duplicates win.  It runs on its
+  // own thread beside the scatter maps and the merges (it writes only amap
+  // and reads the structure).
+  // Common case: every column's rows are strictly increasing and none lies
+  // above the diagonal, so no two entries share a pair; each entry then
+  // finds its row in its target supernode's structure by itself (the
+  // diagonal block directly, the rows below by binary search), on several
+  // threads over column ranges of equal entry counts.
+  // Otherwise entries are bucketed by their permuted column (stable, in
+  // entry order); each supernode marks the positions of its structure rows
+  // once, so every entry finds its row in O(1).  Two entries land on the same
+  // arena offset exactly when they share the permuted pair, i.e. the bucket
+  // and the row, and the earlier one is dropped.
   bool amap_error = false;
   std::exception_ptr amap_exc;
   std::thread amap_thread([&]() {
     try {
+    const Ptr nnz = colptr[n] - baseval;
+    p.amap.resize(nnz);
+    {
+      const int nth = (int)std::max<unsigned>(1, std::min<unsigned>(8, std::thread::hardware_concurrency()));
+      std::vector<Int> split(nth + 1, n);
+      split[0] = 0;
+      for (int t = 1; t < nth; ++t)
+        split[t] = (Int)(std::lower_bound(colptr, colptr + n, (Ptr)(nnz * t / nth) + baseval) - colptr);
+      std::vector<char> unclean(nth, 0), missing(nth, 0);
+      auto work = [&](int t) {
+        for (Int c = split[t]; c < split[t + 1]; ++c) {
+          const int32_t pcol = s.ord.invp[c] - 1;
+          Idx prev = -1;
+          for (Ptr q = colptr[c] - baseval; q < colptr[c + 1] - baseval; ++q) {
+            const Idx r = rowind[q] - baseval;
+            if (r < c || r <= prev) {
+              unclean[t] = 1;
+              return;
+            }
+            prev = r;
+            const int32_t prow = s.ord.invp[r] - 1;
+            const int32_t pr = std::max(prow, pcol), pc = std::min(prow, pcol);
+            const SnInfo& sj = p.sn[row_sup[pc]];
+            int32_t pos = pr - sj.fc;
+            if (pos >= sj.w) {
+              const int32_t* b = &p.lindx0[sj.lx] + sj.w;
+              const int32_t* e = &p.lindx0[sj.lx] + sj.r;
+              const int32_t* f = std::lower_bound(b, e, pr);
+              if (f == e || *f != pr) {
+                missing[t] = 1;
+                return;
+              }
+              pos = (int32_t)(f - &p.lindx0[sj.lx]);
+            }
+            p.amap[q] = sj.off + (int64_t)(pc - sj.fc) * sj.r + pos;
+          }
+        }
+      };
+      std::vector<std::thread> pool;
+      for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+      work(0);
+      for (auto& th : pool) th.join();
+      bool any_unclean = false;
+      for (int t = 0; t < nth; ++t) {
+        any_unclean |= unclean[t] != 0;
+        amap_error |= missing[t] != 0;
+      }
+      if (!any_unclean || amap_error) return;
+    }
     std::vector<int32_t> apos(n, -1);
     {
-      const Ptr nnz = colptr[n] - baseval;
-      p.amap.assign(nnz, -1);
+      std::fill(p.amap.begin(), p.amap.end(), -1);
       std::vector<Ptr> bstart(n + 1, 0), order(nnz);
       std::vector<int32_t> trow_of(nnz);
       for (Int c = 0; c < n; ++c)
@@ -125,11 +179,9 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
   // (the supernode of column lindx[w+b]) the rows a >= first row of that
   // target group are all in struct(J); their positions there (rel) are
   // found on the device from one RelGroup per group.
-  p.tcol.assign(upd, 0);
-  p.tmap.assign(upd, 0);
+  p.upd_size = upd;
   p.rel_groups.clear();
   p.rel_size = 0;
-  std::vector<int32_t> pos(n, -1);
   for (Int I = 0; I < ns; ++I) {
     const SnInfo& si = p.sn[I];
     const int32_t* st = &p.lindx0[si.lx];
@@ -137,18 +189,14 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     int32_t b = 0;
     while (b < m) {
       int32_t J = row_sup[st[si.w + b]];
-      const SnInfo& sj = p.sn[J];
-      // group = consecutive off-diagonal rows falling in J's columns
+      // group = consecutive off-diagonal rows falling in J's columns (the
+      // columns of J are consecutive numbers: the group ends at the first
+      // row past J's last column)
       int32_t gb = b;
-      while (b < m && row_sup[st[si.w + b]] == J) ++b;
-      const int64_t base = p.rel_size;
-      p.rel_groups.push_back({base, I, J, gb, 0});
+      const int32_t jend = p.sn[J].fc + p.sn[J].w;
+      b = (int32_t)(std::lower_bound(st + si.w + b, st + si.r, jend) - (st + si.w));
+      p.rel_groups.push_back({p.rel_size, I, J, gb, b});
       p.rel_size += m - gb;
-      for (int32_t bb = gb; bb < b; ++bb) {
-        int32_t col = st[si.w + bb] - sj.fc;
-        p.tcol[si.upd + bb] = sj.off + (int64_t)col * sj.r;
-        p.tmap[si.upd + bb] = base - gb;
-      }
     }
   }
 
@@ -172,6 +220,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     const int64_t t = ceil_div(p.sn[I].r - p.sn[I].w, kTile);
     level_tiles[p.level[I]] += t * (t + 1) / 2;
   }
+  std::vector<int32_t> pos(n, -1);
   p.deferred_into.assign(ns, -1);
   std::vector<int32_t> merge_of(ns, 0);  // per supernode: 1 + index into merges, 0 = none
   std::vector<int32_t> ncut(ns, 0);      // per deferred supernode: n1
@@ -320,81 +369,75 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
       if (tr.count) p.factor_seq.push_back(tr);
       if (ur.count) p.factor_seq.push_back(ur);
     }
-    LaunchRange sr{2, (int64_t)p.schur_tasks.size(), 0};
-    sr.first_w = (int64_t)p.schur_wide.size();
+    // Schur tasks: counted here, written on the device (schur_emit)
+    LaunchRange sr{2, p.n_schur, 0};
+    sr.first_w = p.n_schur_wide;
+    const size_t emit_first = p.schur_emit.size();
+    int nqmin = INT_MAX, nqmax = 0;
     for (Int I : sns) {
       const SnInfo& si = p.sn[I];
       int64_t m = si.r - si.w;
       if (m <= 0) continue;
-      int64_t nt = ceil_div(m, kTile);
+      SchurEmit e{};
+      e.I = I;
+      e.nt = (int32_t)ceil_div(m, kTile);
+      e.ntw = (int32_t)ceil_div(m, 2 * kTile);
       if (p.deferred_into[I] >= 0) {
         // only the output columns in the parent's columns; the rest is
         // computed by the parent's merged update
         const int32_t n1 = ncut[I];
-        const int64_t ntj = ceil_div(n1, kTile);
-        for (int64_t ti = 0; ti < nt; ++ti)
-          for (int64_t tj = 0; tj <= ti && tj < ntj; ++tj)
-            p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, -n1});
-        for (int64_t ti = 0; ti < ceil_div(m, 2 * kTile); ++ti)
-          for (int64_t tj = 0; tj <= 2 * ti + 1 && tj < ntj; ++tj)
-            p.schur_wide.push_back({I, (int32_t)ti, (int32_t)tj, -n1});
+        e.ntj = (int32_t)ceil_div(n1, kTile);
+        e.d = -n1;
         p.schur_entries += (int64_t)n1 * m - (int64_t)n1 * (n1 - 1) / 2;
       } else {
-        for (int64_t ti = 0; ti < nt; ++ti)
-          for (int64_t tj = 0; tj <= ti; ++tj)
-            p.schur_tasks.push_back({I, (int32_t)ti, (int32_t)tj, merge_of[I]});
-        for (int64_t ti = 0; ti < ceil_div(m, 2 * kTile); ++ti)
-          for (int64_t tj = 0; tj <= 2 * ti + 1 && tj < nt; ++tj)
-            p.schur_wide.push_back({I, (int32_t)ti, (int32_t)tj, merge_of[I]});
+        e.ntj = e.nt;
+        e.d = merge_of[I];
         p.schur_entries += m * (m + 1) / 2;
       }
+      e.first[0] = p.n_schur;
+      e.first[2] = p.n_schur_wide;
+      p.n_schur += schur_tiles_before(e.nt, e.ntj);
+      p.n_schur_wide += schur_wide_before(e.ntw, e.ntj);
+      int nq = (int)ceil_div(si.w, kSchurKC);
+      if (e.d > 0) nq += (int)ceil_div(p.merges[e.d - 1].w, kSchurKC);
+      nqmin = std::min(nqmin, nq);
+      nqmax = std::max(nqmax, nq);
+      e.nq = nq;
+      p.schur_emit.push_back(e);
       // algorithmic flops stay those of the supernode's whole update, wherever
       // its deferred part is computed
       p.schur_flops += (double)si.w * (double)m * (double)(m + 1);
     }
-    sr.count = (int64_t)p.schur_tasks.size() - sr.first;
-    sr.count_w = (int64_t)p.schur_wide.size() - sr.first_w;
+    sr.count = p.n_schur - sr.first;
+    sr.count_w = p.n_schur_wide - sr.first_w;
+    sr.nqmin = sr.count ? nqmin : 0;
+    // longest-first order: the level's supernodes by k chunks, longest first,
+    // stable (a counting sort on the chunk count, a small integer), each
+    // supernode's tiles in their plan order
+    {
+      std::vector<int64_t> cnt(nqmax + 2, 0);
+      for (size_t q = emit_first; q < p.schur_emit.size(); ++q) ++cnt[nqmax - p.schur_emit[q].nq + 1];
+      for (int i = 1; i <= nqmax + 1; ++i) cnt[i] += cnt[i - 1];
+      std::vector<size_t> order(p.schur_emit.size() - emit_first);
+      for (size_t q = emit_first; q < p.schur_emit.size(); ++q)
+        order[cnt[nqmax - p.schur_emit[q].nq]++] = q;
+      int64_t f1 = sr.first, f3 = sr.first_w;
+      for (size_t q : order) {
+        SchurEmit& e = p.schur_emit[q];
+        e.first[1] = f1;
+        e.first[3] = f3;
+        f1 += schur_tiles_before(e.nt, e.ntj);
+        f3 += schur_wide_before(e.ntw, e.ntj);
+      }
+    }
     if (sr.count && sr.count < kSplitTarget) {
       // few output tiles: each task's k chunks are split over S workgroups
       // (partial sums meet in the atomic scatter), S <= the fewest chunks of
       // any task of the level so that no part is empty
-      int nqmin = INT_MAX;
-      for (int64_t q = sr.first; q < sr.first + sr.count; ++q) {
-        const Task4& t = p.schur_tasks[q];
-        int nq = (int)ceil_div(p.sn[t.a].w, kSchurKC);
-        if (t.d > 0) nq += (int)ceil_div(p.merges[t.d - 1].w, kSchurKC);
-        nqmin = std::min(nqmin, nq);
-      }
       sr.aux = (int)std::max<int64_t>(1, std::min<int64_t>(nqmin, kSplitTarget / sr.count));
     }
     if (sr.count) p.factor_seq.push_back(sr);
     for (size_t q = level_first; q < p.factor_seq.size(); ++q) p.factor_seq[q].level = L;
-  }
-  // longest-first copy of every Schur range (k chunks of the panel and of a
-  // merged child)
-  // (a stable counting sort on the chunk count, a small integer)
-  p.schur_tasks_lpt.resize(p.schur_tasks.size());
-  p.schur_wide_lpt.resize(p.schur_wide.size());
-  {
-    auto chunks = [&](const Task4& t) {
-      int nq = (int)ceil_div(p.sn[t.a].w, kSchurKC);
-      if (t.d > 0) nq += (int)ceil_div(p.merges[t.d - 1].w, kSchurKC);
-      return nq;
-    };
-    std::vector<int64_t> cnt;
-    auto longest_first = [&](const std::vector<Task4>& src, std::vector<Task4>& dst, int64_t first, int64_t count) {
-      int maxq = 0;
-      for (int64_t q = first; q < first + count; ++q) maxq = std::max(maxq, chunks(src[q]));
-      cnt.assign(maxq + 2, 0);
-      for (int64_t q = first; q < first + count; ++q) ++cnt[maxq - chunks(src[q]) + 1];
-      for (int i = 1; i <= maxq + 1; ++i) cnt[i] += cnt[i - 1];
-      for (int64_t q = first; q < first + count; ++q) dst[first + cnt[maxq - chunks(src[q])]++] = src[q];
-    };
-    for (const LaunchRange& r : p.factor_seq) {
-      if (r.kind != 2) continue;
-      longest_first(p.schur_tasks, p.schur_tasks_lpt, r.first, r.count);
-      longest_first(p.schur_wide, p.schur_wide_lpt, r.first_w, r.count_w);
-    }
   }
 
   // ---- solve task lists ----

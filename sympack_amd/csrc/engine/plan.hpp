@@ -67,11 +67,43 @@ struct MergeInfo {         // mirrored on the device
 // One run of consecutive off-diagonal rows [gb, ge) of supernode I whose
 // rows are columns of the same ancestor J.  The relative indices of all of
 // I's off-diagonal rows a >= gb inside struct(J) are
-// rel[base + a - gb] (a in [gb, m_I)); they are filled on the device from
-// this table (binary search of each row in struct(J)).
+// rel[base + a - gb] (a in [gb, m_I)); they, and the scatter bases tcol /
+// tmap of the rows [gb, ge), are filled on the device from this table
+// (binary search of each row in struct(J)).
 struct RelGroup {          // mirrored on the device
   int64_t base;
-  int32_t I, J, gb, pad;
+  int32_t I, J, gb, ge;
+};
+
+// Host and device code (the plan is counted on the host, the task lists are
+// written on the device by the same formulas).
+#if defined(__HIPCC__)
+#define SPK_HD __host__ __device__
+#else
+#define SPK_HD
+#endif
+
+// Schur output tiles of one supernode with m off-diagonal rows: 64 x 64
+// tiles (ti, tj), tj <= ti, tj < ntj (ntj = the column-tile count, fewer for
+// a deferred supernode's column cut), in order ti then tj; 128 x 64 tiles
+// (ti, tj), ti < ceil(m / 128), tj <= 2 ti + 1, tj < ntj.  *_before(ti, ntj):
+// tiles of the rows before row tile ti.
+SPK_HD inline int64_t schur_tiles_before(int64_t ti, int64_t ntj) {
+  return ti <= ntj ? ti * (ti + 1) / 2 : ntj * (ntj + 1) / 2 + (ti - ntj) * ntj;
+}
+SPK_HD inline int64_t schur_wide_before(int64_t ti, int64_t ntj) {
+  const int64_t c = ntj / 2;  // row tiles whose 2 ti + 2 columns all lie below the cap
+  return ti <= c ? ti * (ti + 1) : c * (c + 1) + (ti - c) * ntj;
+}
+
+// Per supernode with a Schur update: where its tasks {sn, ti, tj, d} go in
+// the four Schur task arrays (64 x 64 tiles in plan order and longest first,
+// 128 x 64 tiles in plan order and longest first).  nt = ceil(m / 64) row
+// tiles, ntw = ceil(m / 128).
+struct SchurEmit {         // mirrored on the device
+  int64_t first[4];
+  int32_t I, nt, ntw, ntj;
+  int32_t d, nq, pad1, pad2;  // nq: k chunks of each task (panel and merged child)
 };
 
 struct LaunchRange {
@@ -84,9 +116,10 @@ struct LaunchRange {
   int aux = 0;       // kinds 0, 7, 9, 3, 6: diagonal-block bucket (diag_bucket);
                      // kind 2: k-chunk split factor (0 or 1 = none)
   int level = 0;     // etree level of the launch (factor sequence)
-  int alt = 0;       // kind 2: bit 0 = tasks in longest-first order (schur_tasks_lpt /
-                     // schur_wide_lpt), bit 1 = 128 x 64 tiles (k_schur_wide)
+  int alt = 0;       // kind 2: bit 0 = tasks in longest-first order, bit 1 = 128 x 64
+                     // tiles (k_schur_wide)
   int64_t first_w = 0, count_w = 0;  // kind 2: the level's 128 x 64 tile tasks
+  int nqmin = 0;     // kind 2: fewest k chunks of any task of the range
 };
 
 struct Plan {
@@ -98,11 +131,10 @@ struct Plan {
   int32_t nlevels = 0;
   // Schur scatter maps (per off-diagonal row b of supernode I):
   // target address of element (a,b) = tcol[upd+b] + rel[tmap[upd+b] + a].
-  // rel (rel_size entries) exists on the device only, built from rel_groups.
-  std::vector<int64_t> tcol;
-  std::vector<int64_t> tmap;
+  // tcol, tmap (upd_size entries each) and rel (rel_size entries) exist on
+  // the device only, built from rel_groups.
   std::vector<RelGroup> rel_groups;
-  int64_t rel_size = 0;
+  int64_t upd_size = 0, rel_size = 0;
   // assembly: arena offset of every caller CSC entry (-1 = superseded duplicate)
   std::vector<int64_t> amap;
   std::vector<int32_t> perm0;      // perm0[new] = old, 0-based
@@ -113,17 +145,16 @@ struct Plan {
   std::vector<int64_t> inv_base;   // per supernode: first inverse tile (tile t of 16-column block t)
   std::vector<Task4> inpanel_tasks;// {sn, step, ti, tj}
   std::vector<Task4> fused_tasks;  // {sn, row tiles, row_tile, inverse-tile offset}: block factor + TRSM
-  // {sn, ti, tj, d}: 64x64 output tiles; d = 0 plain, d > 0 the update also
-  // carries merges[d-1] (a deferred child), d < 0 only output columns b < -d
-  // (parent columns of a deferred supernode)
-  std::vector<Task4> schur_tasks;
-  // the same tasks with every launch range's tasks ordered by their k
-  // chunks, longest first (stable): an alternative task order for the
-  // autotuner (workgroups are dispatched in index order)
-  std::vector<Task4> schur_tasks_lpt;
-  // the same updates as 128 x 64 output tiles {sn, 128-row tile, 64-column
-  // tile, d} (64 tj <= 128 ti + 127), plain and longest-first per range
-  std::vector<Task4> schur_wide, schur_wide_lpt;
+  // Schur tasks {sn, ti, tj, d}: 64x64 output tiles; d = 0 plain, d > 0 the
+  // update also carries merges[d-1] (a deferred child), d < 0 only output
+  // columns b < -d (parent columns of a deferred supernode).  Four arrays,
+  // written on the device from schur_emit: 64 x 64 tiles in plan order; the
+  // same with every launch range's supernodes ordered by their k chunks,
+  // longest first (stable: an alternative task order for the autotuner,
+  // workgroups are dispatched in index order); 128 x 64 tiles {sn, 128-row
+  // tile, 64-column tile, d} in plan order and longest first.
+  int64_t n_schur = 0, n_schur_wide = 0;
+  std::vector<SchurEmit> schur_emit;
   std::vector<MergeInfo> merges;
   std::vector<int32_t> mmap;
   std::vector<int32_t> deferred_into;  // per supernode: parent absorbing its update, -1 = none
