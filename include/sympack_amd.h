@@ -83,13 +83,15 @@ typedef struct spk_options {
                          lose accuracy).  0: block substitution, four launches per level */
   int autotune;       /* 1 (default): the first spk_factorize of a structure with at least 1e9
                          Schur-update flops times every etree level's Schur launch under each
-                         output tile shape (64 x 64 / 128 x 64), task order (plan / longest
-                         first) and k split (1-8 workgroups per tile) on the device and keeps
-                         the fastest (spk_get_autotune);
-                         0: the plan's heuristic.  No reference counterpart */
-  int schur_tile;     /* 0 (default): Schur output tiles as the autotuner (or, untuned, the plan)
-                         chooses; 64: 64 x 64 tiles everywhere; 128: 128 x 64 tiles everywhere
-                         (no tuning); other values are rejected by spk_init */
+                         kernel (64 x 64 tiles staged through LDS, 128 x 64 tiles, 64 x 64 tiles
+                         with one wave per tile), task order (plan / longest first) and k split
+                         (1-8 workgroups per tile) on the device and keeps the fastest
+                         (spk_get_autotune); 0: the plan's heuristic.  No reference
+                         counterpart */
+  int schur_kernel;   /* 0 (default): the Schur-update kernel the autotuner (or, untuned, the
+                         plan) chooses per level; 1: 64 x 64 tiles staged through LDS, 2: 128 x
+                         64 tiles, 3: 64 x 64 tiles with one wave per tile, on every level (no
+                         tuning); other values are rejected by spk_init */
 } spk_options;
 
 typedef struct spk_stats {
@@ -143,7 +145,8 @@ int spk_get_launch_levels(spk_handle h, int32_t* levels, int max);
 /* The autotuner's choice per Schur launch (one per etree level with updates)
  * after the first factorization: level, k split (workgroups per tile), launch
  * variant (bit 0: tasks longest first instead of plan order; bit 1: 128 x 64
- * output tiles instead of 64 x 64), best time of the default configuration
+ * output tiles; bit 2: 64 x 64 tiles with one wave per tile; neither: 64 x 64
+ * tiles staged through LDS), best time of the default configuration
  * and of the chosen one (ms); returns the number of records (<= max), 0 when
  * nothing was tuned.  No reference counterpart. */
 int spk_get_autotune(spk_handle h, int32_t* level, int32_t* split, int32_t* order, double* ms_default,

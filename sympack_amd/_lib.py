@@ -38,7 +38,7 @@ class SpkOptions(ctypes.Structure):
         ("refine", ctypes.c_int),
         ("solve_inverse", ctypes.c_int),
         ("autotune", ctypes.c_int),
-        ("schur_tile", ctypes.c_int),
+        ("schur_kernel", ctypes.c_int),
     ]
 
 

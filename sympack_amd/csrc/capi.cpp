@@ -84,9 +84,9 @@ spk::EngineOptions to_engine(const spk_options& o, int n_hint) {
   e.refine = o.refine;
   e.solve_inverse = o.solve_inverse;
   e.autotune = o.autotune;
-  if (o.schur_tile != 0 && o.schur_tile != 64 && o.schur_tile != 128)
-    throw std::logic_error("schur_tile must be 0, 64 or 128, got " + std::to_string(o.schur_tile));
-  e.schur_tile = o.schur_tile;
+  if (o.schur_kernel < 0 || o.schur_kernel > 3)
+    throw std::logic_error("schur_kernel must be 0, 1, 2 or 3, got " + std::to_string(o.schur_kernel));
+  e.schur_kernel = o.schur_kernel;
   return e;
 }
 }  // namespace
@@ -146,7 +146,7 @@ void spk_default_options(spk_options* o) {
   o->refine = 0;
   o->solve_inverse = 1;
   o->autotune = 1;
-  o->schur_tile = 0;
+  o->schur_kernel = 0;
 }
 
 int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
@@ -227,8 +227,8 @@ int spk_options_from_args(spk_options* o, int argc, const char* const* argv) {
         o->solve_inverse = std::atoi(val(1).c_str()) != 0;
       } else if (k == "-autotune") {
         o->autotune = std::atoi(val(1).c_str()) != 0;
-      } else if (k == "-schur_tile") {
-        o->schur_tile = std::atoi(val(1).c_str());
+      } else if (k == "-schur_kernel") {
+        o->schur_kernel = std::atoi(val(1).c_str());
       } else if (k == "-dataflow") {
         o->solve_dataflow = std::atoi(val(1).c_str()) != 0;
       }

@@ -824,9 +824,10 @@ void Engine::launch_diagtrsm(int bucket, bool ldl, unsigned g, const Task4* t, T
 #undef SPK_FUSED
 }
 
-// One Schur launch of range r: 64 x 64 (k_update) or 128 x 64
-// (k_schur_wide) tiles (alt bit 1), plan or longest-first task order (alt
-// bit 0), each tile's k chunks split over ks workgroups.
+// One Schur launch of range r: 64 x 64 tiles staged through LDS (k_update),
+// 128 x 64 tiles (k_schur_wide, alt bit 1) or 64 x 64 tiles with one wave per
+// tile (k_schur_w1, alt bit 2), plan or longest-first task order (alt bit 0),
+// each tile's k chunks split over ks workgroups (waves for k_schur_w1).
 template <typename T>
 void Engine::launch_schur(const LaunchRange& r, int alt, int ks, T* arena, hipStream_t st) {
   const SnInfo* sn = (const SnInfo*)d_sn_;
@@ -838,7 +839,15 @@ void Engine::launch_schur(const LaunchRange& r, int alt, int ks, T* arena, hipSt
                                               : ((alt & 1) ? TA_SCHUR_LPT : TA_SCHUR)] +
                    (wide ? r.first_w : r.first);
   const unsigned gs = (unsigned)(cnt * ks);
-  if (wide) {
+  // k_schur_w1 addresses panels through 32-bit buffer offsets: below 2 GiB
+  if ((alt & 4) && plan_.max_panel * (int64_t)sizeof(T) >= (int64_t(1) << 31)) alt &= 1;
+  if (alt & 4) {  // one wave per (task, part): four per workgroup
+    const unsigned gw = (unsigned)((cnt * ks + 3) / 4);
+    if (ldl)
+      dev::k_schur_w1<T, true><<<gw, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+    else
+      dev::k_schur_w1<T, false><<<gw, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
+  } else if (wide) {
     if (ldl)
       dev::k_schur_wide<T, true><<<gs, 256, 0, st>>>(sn, t, arena, d_tcol_, d_tmap_, d_rel_, d_merges_, d_mmap_, cnt, ks);
     else
@@ -940,6 +949,7 @@ void Engine::autotune_schur(T* arena) {
     if (!warm) {  // first launches of the kernels in this process: load them untimed
       launch(r, 0, std::max(1, r.aux));
       if (r.count_w > 0) launch(r, 2, 1);
+      launch(r, 4, 1);
       warm = true;
     }
     TuneRecord rec;
@@ -949,7 +959,7 @@ void Engine::autotune_schur(T* arena) {
     rec.alt0 = rec.alt = r.alt;
     rec.ms0 = rec.ms = time_of(r, r.alt, rec.split0);
     rec.candidates = 1;
-    for (int alt = 0; alt < 4; ++alt)
+    for (int alt = 0; alt < 6; ++alt)
       for (int ks : {1, 2, 3, 4, 6, 8}) {
         const int64_t tiles = (alt & 2) ? r.count_w : r.count;
         if (tiles == 0 || ks > nqmin || tiles * ks > 8192) continue;
@@ -976,10 +986,11 @@ int Engine::factorize() {
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(d_info_, &big, sizeof(int), hipMemcpyHostToDevice, stream_));
   ginv_ready_ = false;
-  if (opt_.schur_tile != 0 && !tuned_) {
-    // a fixed tile shape for every Schur launch, plan order and k split
+  if (opt_.schur_kernel != 0 && !tuned_) {
+    // one Schur kernel for every launch, plan order and k split
     for (LaunchRange& r : plan_.factor_seq)
-      if (r.kind == 2) r.alt = (opt_.schur_tile == 128 && r.count_w > 0) ? 2 : 0;
+      if (r.kind == 2)
+        r.alt = opt_.schur_kernel == 3 ? 4 : (opt_.schur_kernel == 2 && r.count_w > 0) ? 2 : 0;
     tuned_ = true;
     destroy_graphs();
   }

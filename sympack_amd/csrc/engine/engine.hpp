@@ -40,8 +40,8 @@ struct EngineOptions {
   int refine = 0;         // 10*ordflag + altflag of the supernode reordering, 0 = none
   int autotune = 1;       // time the Schur launch configurations of every level on the first
                           // factorization of a plan with at least kAutotuneMinFlops Schur flops
-  int schur_tile = 0;     // 0: tuner / plan default; 64: 64 x 64 tiles everywhere; 128: 128 x 64
-                          // tiles everywhere (no tuning when set)
+  int schur_kernel = 0;   // 0: tuner / plan default; 1, 2, 3: k_update, k_schur_wide, k_schur_w1
+                          // on every level (no tuning when set)
   std::vector<Int> user_perm;
 };
 

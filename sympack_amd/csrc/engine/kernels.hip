@@ -1231,6 +1231,251 @@ __global__ __launch_bounds__(NT) void k_schur_wide(const SnInfo* __restrict__ sn
 }
 
 // ---------------------------------------------------------------------------
+// Schur update with one wave per 64 x 64 output tile (the autotuner's
+// candidate without LDS staging).  The same C = -P_a P_b^T (LDL:
+// -P_a D P_b^T) as k_update<T,1,LDL>, but each wave owns a whole tile as 4 x 4
+// MFMA 16x16x4 blocks (16 independent accumulators: one wave keeps the
+// matrix core busy on its own) and streams its operands from memory straight
+// into registers, two k steps of 4 ahead of the products, so there is no
+// barrier: the four waves of a workgroup are four independent (task, part)
+// items, and two waves per SIMD (about 200 registers each) overlap one
+// tile's scatter with another's products.  Operand rows past the panel are
+// clamped, not zeroed (they only reach outputs a >= m or b >= m, which are
+// never stored); k past the width and a merged child's missing rows are
+// zeroed.  A diagonal tile loads one row set for both sides and skips the
+// six blocks above the diagonal; edge and deferred tiles skip the blocks
+// past m or past the column cut.  Scatter as in k_update: column bases from
+// a wave-private LDS copy of tcol / tmap, relative rows from rel, atomics.
+// ---------------------------------------------------------------------------
+template <typename T, bool LDL, int SHAPE>
+__device__ __forceinline__ void w1_products(typename Mfma<T>::v4 (&acc)[4][4], const T (&ra)[4], const T (&rb)[4],
+                                            T d, int nbi, int nbj) {
+  // SHAPE 0: all 16 blocks; 1: diagonal tile, blocks with i >= j (rb = ra);
+  // 2: blocks i < |nbi|, j < nbj, and i >= j when nbi < 0 (a diagonal tile)
+  T bf[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bf[j] = LDL ? rb[j] * d : rb[j];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (SHAPE == 1 && i < j) continue;
+      if (SHAPE == 2) {
+        const bool dg = nbi < 0;
+        const int ni = dg ? -nbi : nbi;
+        if (i >= ni || j >= nbj || (dg && i < j)) continue;  // uniform
+      }
+      acc[j][i] = Mfma<T>::mma(bf[j], ra[i], acc[j][i]);
+    }
+}
+
+// Loads one element through a buffer resource; offsets at or past the
+// resource's size read 0.
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+// Per-wave body of k_schur_w1, specialized on the tile shape (SHAPE as in
+// w1_products) and on whether the task carries a merged child's k columns.
+// Operands come through buffer resources spanning the panel (and the merged
+// child's panel): a k column past the width, a child's missing row (offset
+// 0xffffffff) and, for LDL, d_k past the width all read 0, so the products
+// need no masks.  Panels are below 2 GiB (the launch checks it).
+template <typename T, bool LDL, int SHAPE, bool MERGED>
+__device__ __forceinline__ void w1_body(const SnInfo& s, const Task4& t, T* __restrict__ arena,
+                                        const int32_t* __restrict__ rel, const MergeInfo* __restrict__ merges,
+                                        const int32_t* __restrict__ mmap, const int64_t* __restrict__ sTc,
+                                        const int64_t* __restrict__ sTm, int g_lo, int g_hi, int ng1, int nbi,
+                                        int nbj) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr uint32_t TB = sizeof(T);
+  const int lane = threadIdx.x & 63;
+  const uint32_t ld = (uint32_t)s.r;
+  const int K = s.w, m = s.r - s.w, ti = t.b, tj = t.c;
+  const int r16 = lane & 15, kq = lane >> 4;
+  constexpr bool DIAG = SHAPE == 1;
+  const __amdgpu_buffer_rsrc_t rP = make_rsrc(arena + s.off, (uint32_t)s.r * (uint32_t)s.w * TB);
+  // byte offsets of this lane's operand rows in each 16-row block (clamped
+  // into the panel: rows past m only reach outputs that are never stored)
+  uint32_t aoff[4], boff[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    aoff[i] = (uint32_t)(s.w + min(ti * TILE + 16 * i + r16, m - 1)) * TB;
+    boff[i] = (uint32_t)(s.w + min(tj * TILE + 16 * i + r16, m - 1)) * TB;
+  }
+  __amdgpu_buffer_rsrc_t rC = rP;
+  uint32_t ld2 = 1;
+  uint32_t aoff2[4] = {0, 0, 0, 0}, boff2[4] = {0, 0, 0, 0};
+  if (MERGED) {
+    const MergeInfo mg = merges[t.d - 1];
+    rC = make_rsrc(arena + mg.off, (uint32_t)mg.ld * (uint32_t)mg.w * TB);
+    ld2 = (uint32_t)mg.ld;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ma = mmap[mg.moff + (aoff[i] / TB - s.w)];
+      const int mb = mmap[mg.moff + (boff[i] / TB - s.w)];
+      aoff2[i] = ma >= 0 ? (uint32_t)ma * TB : 0xffffffffu;
+      boff2[i] = mb >= 0 ? (uint32_t)mb * TB : 0xffffffffu;
+    }
+  }
+  v4 acc[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[j][i] = v4{0, 0, 0, 0};
+
+  // operand registers of one k step (4 k columns: lane l holds k = l / 16)
+  struct Ops {
+    T a[4], b[4], d;
+  };
+  // step g of this part; steps at or past g_hi read zeros (offsets past the
+  // resource), so the loop below can always run whole pairs of steps
+  auto fetch = [&](int g, Ops& o) {
+    const bool child = MERGED && g >= ng1;
+    const uint32_t k = (uint32_t)(4 * (child ? g - ng1 : g) + kq);
+    const __amdgpu_buffer_rsrc_t r = child ? rC : rP;
+    const uint32_t col = g < g_hi ? k * (child ? ld2 : ld) * TB : 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t ao = child ? aoff2[i] : aoff[i];
+      o.a[i] = bload<T>(r, (ao == 0xffffffffu || col == 0xffffffffu) ? 0xffffffffu : col + ao);
+    }
+    if (!DIAG) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t bo = child ? boff2[i] : boff[i];
+        o.b[i] = bload<T>(r, (bo == 0xffffffffu || col == 0xffffffffu) ? 0xffffffffu : col + bo);
+      }
+    }
+    if (LDL) o.d = bload<T>(r, col == 0xffffffffu ? col : col + k * TB);
+  };
+  auto compute = [&](const Ops& o) {
+    w1_products<T, LDL, SHAPE>(acc, o.a, DIAG ? o.a : o.b, LDL ? o.d : T(1), nbi, nbj);
+  };
+  // the k loop over triples of steps: steps g + 1 and g + 2 are in flight
+  // during the products of step g (a count that is not a multiple of three
+  // multiplies one or two steps of zeros); the scheduling barriers keep each
+  // step's loads ahead of the products they overlap
+  Ops o0, o1, o2;
+  fetch(g_lo, o0);
+  fetch(g_lo + 1, o1);
+  for (int g = g_lo; g < g_hi; g += 3) {
+    fetch(g + 2, o2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(o0);
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(g + 3, o0);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(o1);
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(g + 4, o1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(o2);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // scatter: bases from LDS, relative rows from rel (defined for rows a >=
+  // the first row of b's target group: clamped, unconditional loads), one
+  // 16-column block at a time
+  const int bcut = t.d < 0 ? -t.d : m;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int32_t rv[4][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bl = 16 * j + Mfma<T>::row(lane, v);
+      const int bc = min(tj * TILE + bl, m - 1);
+      const int64_t tmb = sTm[bl];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ac = min(max(ti * TILE + 16 * i + r16, bc), m - 1);
+        rv[v][i] = rel[tmb + ac];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bl = 16 * j + Mfma<T>::row(lane, v);
+      const int b = tj * TILE + bl;
+      const int64_t tc = sTc[bl];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int a = ti * TILE + 16 * i + r16;
+        if (b < bcut && a < m && a >= b) atomic_add(arena + tc + rv[v][i], -acc[j][i][v]);
+      }
+    }
+  }
+}
+
+template <typename T, bool LDL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_schur_w1(const SnInfo* __restrict__ sn, const Task4* __restrict__ tasks, T* __restrict__ arena,
+                const int64_t* __restrict__ tcol, const int64_t* __restrict__ tmap,
+                const int32_t* __restrict__ rel, const MergeInfo* __restrict__ merges,
+                const int32_t* __restrict__ mmap, int64_t ntasks, int ksplit) {
+  constexpr int KC = kSchurKC;
+  __shared__ int64_t sTc[NT / 64][TILE], sTm[NT / 64][TILE];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t gi = (int64_t)blockIdx.x * (NT / 64) + wv;
+  if (gi >= ntasks * ksplit) return;  // the whole wave
+  const int64_t task = gi / ksplit;
+  const int part = (int)(gi - task * ksplit);
+  const Task4 t = tasks[task];
+  const SnInfo s = sn[t.a];
+  const int K = s.w, m = s.r - s.w, ti = t.b, tj = t.c;
+  int K2 = 0;
+  if (t.d > 0) K2 = merges[t.d - 1].w;
+  // scatter bases of the tile's 64 output columns (wave-private LDS: the
+  // wave's own LDS accesses complete in order, no barrier)
+  {
+    const int bc = min(tj * TILE + lane, m - 1);
+    sTc[wv][lane] = tcol[s.upd + bc];
+    sTm[wv][lane] = tmap[s.upd + bc];
+  }
+  // k runs in MFMA steps of 4; chunks of KC = 32 (8 steps) are the unit of
+  // the k split, as in k_update
+  constexpr int SPC = KC / 4;
+  const int nq1 = (K + KC - 1) / KC, nq = nq1 + (K2 + KC - 1) / KC;
+  const int q_lo = (int)((int64_t)part * nq / ksplit), q_hi = (int)((int64_t)(part + 1) * nq / ksplit);
+  if (q_lo >= q_hi) return;
+  const int ng1 = (K + 3) / 4, ng2 = (K2 + 3) / 4;
+  const int g_lo = q_lo < nq1 ? SPC * q_lo : ng1 + SPC * (q_lo - nq1);
+  const int g_hi = q_hi - 1 < nq1 ? min(SPC * q_hi, ng1) : min(ng1 + SPC * (q_hi - nq1), ng1 + ng2);
+  // blocks of this tile that hold stored outputs
+  const bool diag = ti == tj;
+  const int nbi = min(4, (m - ti * TILE + 15) / 16);
+  const int bcut = t.d < 0 ? -t.d : m;
+  const int nbj = min(4, (min(bcut, m) - tj * TILE + 15) / 16);
+  const int64_t* c = sTc[wv];
+  const int64_t* mm = sTm[wv];
+  if (nbi == 4 && nbj == 4 && !diag) {
+    if (t.d > 0)
+      w1_body<T, LDL, 0, true>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, 4, 4);
+    else
+      w1_body<T, LDL, 0, false>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, 4, 4);
+  } else if (nbi == 4 && nbj == 4) {
+    if (t.d > 0)
+      w1_body<T, LDL, 1, true>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, 4, 4);
+    else
+      w1_body<T, LDL, 1, false>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, 4, 4);
+  } else {
+    const int nbi_enc = diag ? -nbi : nbi;
+    if (t.d > 0)
+      w1_body<T, LDL, 2, true>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, nbi_enc, nbj);
+    else
+      w1_body<T, LDL, 2, false>(s, t, arena, rel, merges, mmap, c, mm, g_lo, g_hi, ng1, nbi_enc, nbj);
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 // Solve.  Vectors are row-major n x nrhs in the permuted numbering.
 // Forward:  X_I = L_II^{-1} Y_I  (k_sdiag_fwd), then Y[struct rows] -= L_off X_I
 //           (k_soff_fwd, atomics).  LDL then scales by D^{-1} (k_scale_d).

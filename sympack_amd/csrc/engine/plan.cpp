@@ -38,6 +38,7 @@ void build_plan(const Symbolic& s, Int n, const Ptr* colptr, const Idx* rowind, 
     off += (int64_t)si.r * si.w;
     upd += si.r - si.w;
     p.max_w = std::max(p.max_w, (int)si.w);
+    p.max_panel = std::max(p.max_panel, (int64_t)si.r * si.w);
   }
   p.arena_elems = off;
 

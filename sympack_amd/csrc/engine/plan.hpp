@@ -117,7 +117,7 @@ struct LaunchRange {
                      // kind 2: k-chunk split factor (0 or 1 = none)
   int level = 0;     // etree level of the launch (factor sequence)
   int alt = 0;       // kind 2: bit 0 = tasks in longest-first order, bit 1 = 128 x 64
-                     // tiles (k_schur_wide)
+                     // tiles (k_schur_wide), bit 2 = one wave per 64 x 64 tile (k_schur_w1)
   int64_t first_w = 0, count_w = 0;  // kind 2: the level's 128 x 64 tile tasks
   int nqmin = 0;     // kind 2: fewest k chunks of any task of the range
 };
@@ -184,6 +184,7 @@ struct Plan {
   double schur_flops = 0, inpanel_flops = 0, diag_flops = 0;
   int64_t schur_entries = 0;       // lower-triangle entries scattered
   int max_w = 0;
+  int64_t max_panel = 0;           // largest panel, r * w elements
 };
 
 // Build the plan from the symbolic analysis and the caller's CSC pattern

@@ -73,7 +73,7 @@ class symPACKOptions:
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
                  profile=False, device=0, merge_updates=1, solve_dataflow=False,
-                 order_refinement_str="NO", solve_inverse=True, autotune=True, schur_tile=0):
+                 order_refinement_str="NO", solve_inverse=True, autotune=True, schur_kernel=0):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()
@@ -95,8 +95,9 @@ class symPACKOptions:
         # time the Schur launch configurations per etree level on the first
         # factorization (structures with >= 1e9 Schur flops)
         self.autotune = autotune
-        # Schur output tile shape: 0 autotuner / plan, 64 (64 x 64), 128 (128 x 64)
-        self.schur_tile = schur_tile
+        # Schur-update kernel on every level: 0 autotuner / plan, 1 64 x 64 tiles staged
+        # through LDS, 2 128 x 64 tiles, 3 64 x 64 tiles with one wave per tile
+        self.schur_kernel = schur_kernel
 
     @classmethod
     def from_args(cls, args):
@@ -115,7 +116,7 @@ class symPACKOptions:
                    merge_updates=int(o.merge_updates), solve_dataflow=bool(o.solve_dataflow),
                    order_refinement_str=refine_name(o.refine),
                    solve_inverse=bool(o.solve_inverse), autotune=bool(o.autotune),
-                   schur_tile=int(o.schur_tile))
+                   schur_kernel=int(o.schur_kernel))
 
     def to_c(self):
         lib = _lib.load()
@@ -140,7 +141,7 @@ class symPACKOptions:
         o.solve_dataflow = 1 if self.solve_dataflow else 0
         o.solve_inverse = 1 if self.solve_inverse else 0
         o.autotune = 1 if self.autotune else 0
-        o.schur_tile = int(self.schur_tile)
+        o.schur_kernel = int(self.schur_kernel)
         o.refine = refine_code(self.order_refinement_str)
         return o
 
@@ -339,7 +340,8 @@ class symPACKMatrix2D:
     def autotune_log(self, max_records=4096):
         """The autotuner's record per Schur launch: dicts with level, split,
         order (bit 0: longest first instead of plan order, bit 1: 128 x 64
-        output tiles instead of 64 x 64), ms_default and ms_best."""
+        output tiles, bit 2: 64 x 64 tiles with one wave per tile), ms_default
+        and ms_best."""
         lv, sp_, od = (np.zeros(max_records, np.int32) for _ in range(3))
         m0, m1 = np.zeros(max_records), np.zeros(max_records)
         cnt = _lib.check(self._lib.spk_get_autotune(self._h, _ptr(lv, ctypes.c_int32), _ptr(sp_, ctypes.c_int32),

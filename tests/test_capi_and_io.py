@@ -186,11 +186,12 @@ def test_unknown_matrix_format():
         sp.DistSparseMatrix.read("/nonexistent", "MATRIX_MARKET")
 
 
-def test_schur_tile_option_is_validated():
+def test_schur_kernel_option_is_validated():
     m = sp.symPACKMatrix2D()
-    for ok in (0, 64, 128):
-        m.Init(sp.symPACKOptions(schur_tile=ok))
-    with pytest.raises(ValueError, match="schur_tile"):
-        m.Init(sp.symPACKOptions(schur_tile=96))
-    o = sp.symPACKOptions.from_args(["-schur_tile", "128", "-autotune", "0"])
-    assert o.schur_tile == 128 and o.autotune is False
+    for ok in (0, 1, 2, 3):
+        m.Init(sp.symPACKOptions(schur_kernel=ok))
+    for bad in (4, -1, 64):
+        with pytest.raises(ValueError, match="schur_kernel"):
+            m.Init(sp.symPACKOptions(schur_kernel=bad))
+    o = sp.symPACKOptions.from_args(["-schur_kernel", "2", "-autotune", "0"])
+    assert o.schur_kernel == 2 and o.autotune is False

@@ -1,6 +1,7 @@
 """The Schur-launch autotuner: on the first factorization of a large enough
-structure every etree level's Schur launch is timed under each output tile
-shape, task order and k split and the fastest is kept.  The tuned factorization must give the
+structure every etree level's Schur launch is timed under each Schur kernel
+(64 x 64 tiles staged through LDS, 128 x 64 tiles, 64 x 64 tiles with one
+wave per tile), task order and k split and the fastest is kept.  The tuned factorization must give the
 oracle's solution (1e-10 relative, the parity bar), and so must the
 refactorizations that replay the recaptured launch sequence; the tuner's
 record must be consistent (one record per Schur launch, the chosen time no
@@ -38,7 +39,7 @@ def test_autotuned_factor_matches_oracle(ldl):
     assert m.stats()["schur_flops"] >= 1e9
     assert len(log) > 0
     for r in log:
-        assert 1 <= r["split"] <= 8 and r["order"] in (0, 1, 2, 3)
+        assert 1 <= r["split"] <= 8 and r["order"] in (0, 1, 2, 3, 4, 5)
         assert 0 < r["ms_best"] <= r["ms_default"] * (1 + 1e-12)
     rng = np.random.default_rng(7)
     B = rng.uniform(-1, 1, (n, 3))
@@ -82,12 +83,13 @@ def test_small_structures_are_not_tuned():
     assert m.autotune_log() == []
 
 
-# 128 x 64 Schur tiles (k_schur_wide) forced on every level: plain updates,
-# merged chain updates (merge_updates=2 makes every qualifying chain child
-# defer into its parent) and deferred column cuts, LL^T and static LDL^T,
-# FP64 and FP32 + refinement.  Parity bar: solutions within 1e-10 relative of
-# the oracle's.
-WIDE_CASES = [
+# Each Schur kernel forced on every level (schur_kernel 2: 128 x 64 tiles,
+# k_schur_wide; 3: 64 x 64 tiles with one wave per tile, k_schur_w1): plain
+# updates, merged chain updates (merge_updates=2 makes every qualifying chain
+# child defer into its parent) and deferred column cuts, LL^T and static
+# LDL^T, FP64 and FP32 + refinement.  Parity bar: solutions within 1e-10
+# relative of the oracle's.
+FORCED_CASES = [
     ("diff3d_16_b24_merged", lambda: M.diffusion_3d_7pt(16, seed=4), 24, 2),
     ("st27_12_b20_merged", lambda: M.stencil_3d_27pt(12, seed=3), 20, 2),
     ("diff3d_24_b150", lambda: M.diffusion_3d_7pt(24, seed=8), 150, 1),
@@ -95,9 +97,10 @@ WIDE_CASES = [
 ]
 
 
+@pytest.mark.parametrize("kernel", [2, 3], ids=["wide", "w1"])
 @pytest.mark.parametrize("ldl", [False, True], ids=["LL", "LDL"])
-@pytest.mark.parametrize("name,gen,maxsize,merge", WIDE_CASES, ids=[c[0] for c in WIDE_CASES])
-def test_wide_schur_tiles_match_oracle(name, gen, maxsize, merge, ldl):
+@pytest.mark.parametrize("name,gen,maxsize,merge", FORCED_CASES, ids=[c[0] for c in FORCED_CASES])
+def test_forced_schur_kernel_matches_oracle(name, gen, maxsize, merge, ldl, kernel):
     n, cp, ri, v = gen()
     A = sp.DistSparseMatrix(n, cp, ri, v)
     rp = sp.RelaxationParameters()
@@ -105,10 +108,10 @@ def test_wide_schur_tiles_match_oracle(name, gen, maxsize, merge, ldl):
         rp.SetMaxSize(maxsize)
     m = sp.symPACKMatrix2D()
     m.Init(sp.symPACKOptions(ordering="NDBOX" if maxsize else "MMD", decomposition="LDL" if ldl else "LL",
-                             relax=rp, merge_updates=merge, schur_tile=128))
+                             relax=rp, merge_updates=merge, schur_kernel=kernel))
     m.SymbolicFactorization(A)
     m.DistributeMatrix(A)
-    print("case %s ldl=%s schur_tile=128" % (name, ldl), flush=True)
+    print("case %s ldl=%s schur_kernel=%d" % (name, ldl, kernel), flush=True)
     m.Factorize()
     rng = np.random.default_rng(9)
     B = rng.uniform(-1, 1, (n, 2))
@@ -121,16 +124,17 @@ def test_wide_schur_tiles_match_oracle(name, gen, maxsize, merge, ldl):
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-10
 
 
-def test_wide_schur_tiles_fp32_ir():
+@pytest.mark.parametrize("kernel", [2, 3], ids=["wide", "w1"])
+def test_forced_schur_kernel_fp32_ir(kernel):
     n, cp, ri, v = M.stencil_3d_27pt(16, seed=2)
     A = sp.DistSparseMatrix(n, cp, ri, v)
     rp = sp.RelaxationParameters()
     rp.SetMaxSize(150)
     m = sp.symPACKMatrix2D()
-    m.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision="FP32_IR", schur_tile=128))
+    m.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision="FP32_IR", schur_kernel=kernel))
     m.SymbolicFactorization(A)
     m.DistributeMatrix(A)
-    print("case 27-point 16^3 FP32_IR schur_tile=128", flush=True)
+    print("case 27-point 16^3 FP32_IR schur_kernel=%d" % kernel, flush=True)
     m.Factorize()
     B = np.random.default_rng(4).standard_normal((n, 3))
     m.Solve(B, 3)
