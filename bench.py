@@ -528,7 +528,8 @@ def main():
                                               (5, "bwd_offdiag"), (6, "bwd_diag"),
                                               (10, "fwd_partitioned_inverse"), (11, "bwd_partitioned_inverse"),
                                               (12, "partitioned_inverse_prep")]},
-        "roofline": {"kernel": "k_update<double,1> (Schur update, FP64 MFMA + atomic scatter)",
+        "roofline": {"kernel": "Schur-update launches (FP64 MFMA + atomic scatter; k_update<double,1>, "
+                               "k_schur_w1 or k_schur_wide per etree level as autotuned)",
                      "timing": "HIP events around each Schur-update launch, %d host-launched "
                                "factorizations after the graph-replayed throughput loop" % args.steps,
                      "bound": "mfma",
