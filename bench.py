@@ -215,6 +215,7 @@ def serving_leg(n_requests, rate, precision, slots, seed=0):
     rep["factor_requests"] = nf
     rep["arrival_rate_per_s"] = rate
     rep["slots"] = slots
+    rep["solve_slots"] = 1 if slots > 1 else 0
     rep["hbm_budget_gb"] = round(budget / 1e9, 1)
     rep["precision"] = precision
     return rep

@@ -12,7 +12,9 @@ indefinite, static LDL^T) matrices identified by a key:
 
 `slots` worker threads pick work as it becomes runnable: a request that
 arrives while others run on the GPU is started as soon as a slot and the
-memory are free.  Every factor owns its own HIP stream, so the work of
+memory are free.  `solve_slots` of them (default 1 when there are several)
+take only solves, so a burst of large factorizations cannot hold every slot
+while short solves on resident factors wait.  Every factor owns its own HIP stream, so the work of
 different requests overlaps on the device.  One request runs on a factor at
 a time; the results do not depend on what else is in flight.
 
@@ -53,9 +55,13 @@ class _Entry:
 
 class SolverService:
     def __init__(self, hbm_budget_bytes=64 << 30, slots=4, max_batch_cols=64,
-                 matrix_factory=symPACKMatrix2D):
+                 matrix_factory=symPACKMatrix2D, solve_slots=None):
         if slots < 1:
             raise ValueError("slots must be >= 1")
+        if solve_slots is None:
+            solve_slots = 1 if slots > 1 else 0
+        if not 0 <= solve_slots < slots:
+            raise ValueError("solve_slots must leave at least one slot for factorizations")
         self.budget = int(hbm_budget_bytes)
         self.max_batch_cols = int(max_batch_cols)
         self._factory = matrix_factory
@@ -69,7 +75,8 @@ class SolverService:
         self.records = []   # (kind, key, n, ncols, t_arrival, t_start, t_done)
         self.evictions = 0
         self.batched_solves = 0
-        self._workers = [threading.Thread(target=self._work, daemon=True, name="spk-slot-%d" % i)
+        self._workers = [threading.Thread(target=self._work, args=(i < solve_slots,), daemon=True,
+                                          name="spk-slot-%d" % i)
                          for i in range(slots)]
         for w in self._workers:
             w.start()
@@ -188,7 +195,7 @@ class SolverService:
             self.evictions += 1
         return self._resident_bytes + need <= self.budget
 
-    def _next_job(self):
+    def _next_job(self, solves_only=False):
         # 1. solves on resident factors (short, latency-critical)
         while self._solve_q:
             e = self._solve_q.popleft()
@@ -200,6 +207,8 @@ class SolverService:
                     cols += item[0].shape[1]
                 e.state = "solving"
                 return ("solve", e, batch)
+        if solves_only:
+            return None
         # 2. factorizations admitted by memory (FIFO; a large one waits for room)
         if self._admit_q:
             e = self._admit_q[0]
@@ -216,12 +225,12 @@ class SolverService:
             return ("analyze", e, None)
         return None
 
-    def _work(self):
+    def _work(self, solves_only):
         while True:
             with self._cv:
                 job = None
                 while not self._closing:
-                    job = self._next_job()
+                    job = self._next_job(solves_only)
                     if job is not None:
                         break
                     self._cv.wait()

@@ -149,3 +149,37 @@ def test_request_stream_shape():
             assert 1000 <= n <= 3e5
         else:
             assert key in seen and 1 <= p[0] <= 32
+
+
+class SlowFactor(DenseFactor):
+    """Factorizations that take long enough to fill every general slot."""
+    gate = threading.Event()
+
+    def Factorize(self):
+        if getattr(self.opts, "orderingStr", "") == "NATURAL":
+            SlowFactor.gate.wait(10.0)
+        super().Factorize()
+
+
+def test_solve_slot_serves_while_factorizations_hold_the_other_slots():
+    """With one of three slots reserved for solves, a solve on a resident
+    factor completes while two factorizations occupy both general slots."""
+    SlowFactor.gate.clear()
+    mats = {("m%d" % i): _mat(i) for i in range(3)}
+    with SolverService(slots=3, solve_slots=1, matrix_factory=SlowFactor) as s:
+        s.submit_factor("m0", mats["m0"]).result(timeout=10)
+        slow = [s.submit_factor(k, mats[k], sp.symPACKOptions(ordering="NATURAL")) for k in ("m1", "m2")]
+        time.sleep(0.05)  # both slow factorizations are running (or queued behind analysis)
+        B = np.ones((mats["m0"].size, 1))
+        x = s.submit_solve("m0", B).result(timeout=5)
+        assert not any(f.done() for f in slow)
+        A = mats["m0"].to_scipy_full().toarray()
+        assert np.allclose(A @ x, B)
+        SlowFactor.gate.set()
+        for f in slow:
+            f.result(timeout=10)
+
+
+def test_solve_slots_must_leave_a_general_slot():
+    with pytest.raises(ValueError, match="solve_slots"):
+        SolverService(slots=2, solve_slots=2, matrix_factory=DenseFactor)
