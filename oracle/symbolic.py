@@ -24,10 +24,9 @@ symbolic elimination (tests/test_symbolic.py); postorder, SortChildren
 supernodes, the relaxed partition's structure (unions of fundamental
 supernodes along parent chains, width caps, every merge of size <= nrelax0
 taken) and the relaxed row structures against a dense boolean elimination
-(tests/test_symbolic_invariants.py).  Parity unpinned: the z-threshold
-decisions of relax_supernodes for merges wider than nrelax0
-(impl:529-566) follow the reference's arithmetic and are checked only
-against the C++ restatement.
+(tests/test_symbolic_invariants.py), and relax_supernodes' merge decisions
+by replaying them on the dense pattern (each merge's zeros counted entry by
+entry, the thresholds applied) in the same test.
 """
 import math
 

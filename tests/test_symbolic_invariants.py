@@ -6,9 +6,12 @@ check runs on both the C++ product chain and the Python oracle
 (oracle/symbolic.py), so a misreading shared by the two restatements of
 ETree.cpp:59-225 and symPACKMatrixBase_impl.hpp:42-609 fails here.
 
-What stays unpinned by construction: the z-threshold decisions of the
-relaxation (symPACKMatrixBase_impl.hpp:529-566) for merges wider than
-nrelax0; only their preconditions are checked."""
+The relaxation's merge decisions (symPACKMatrixBase_impl.hpp:473-609) are
+replayed on the dense pattern: the zeros each merge introduces are counted
+entry by entry (and must equal the reference's closed form), the column
+counts come from the pattern, and the resulting partition must be the
+product's.  The comparison rule itself (thresholds by merged width) is the
+reference's, restated."""
 import numpy as np
 import pytest
 
@@ -127,6 +130,79 @@ def children_of(par):
         if par[v - 1]:
             kids[par[v - 1]].append(v)
     return kids
+
+
+def replay_relaxation(L, par, fund, rp):
+    """Greedy relaxation (symPACKMatrixBase_impl.hpp:473-609) replayed on the
+    dense pattern: fundamental supernodes from the last to the first; one
+    joins the relaxed group that starts right after it when its last
+    column's parent lies in that group, and the merged width and explicit
+    zeros pass the thresholds.  The inputs come from the pattern itself:
+    column counts from L, and the zeros a merge introduces counted entry by
+    entry (columns of the child padded to the merged structure), which must
+    equal the closed form w (cc(parent group's first column) + w - cc(child's
+    first column)) of a nested chain.  As in the reference, zeros of merges
+    taken because the merged width is within nrelax0 are not added to the
+    group's running total.  Returns the relaxed partition (1-based first
+    columns, n + 1 last)."""
+    n = L.shape[0]
+    cc = L.sum(axis=0)
+    nf = len(fund) - 1
+    width = [fund[k + 1] - fund[k] for k in range(nf)]
+    of_col = np.zeros(n + 1, np.int64)
+    for k in range(nf):
+        of_col[fund[k]:fund[k + 1]] = k
+    group_start = list(range(nf))     # representative (first fundamental) of each group
+    ncols = list(width)
+    zeros = [0] * nf
+
+    def rep(k):
+        while group_start[k] != k:
+            k = group_start[k]
+        return k
+
+    for k in range(nf - 1, -1, -1):
+        fc, lc = fund[k], fund[k + 1] - 1
+        pc = par[lc - 1]
+        if pc == 0:
+            continue
+        g = rep(int(of_col[pc]))
+        if g != k + 1:
+            continue
+        w, pw = width[k], ncols[g]
+        pfc = fund[g]
+        W = w + pw
+        # zeros the merge introduces, entry by entry: every child column is
+        # stored over the merged structure (its rows from the parent group's
+        # first column on are the parent's rows)
+        prow = np.nonzero(L[pfc - 1:, pfc - 1])[0] + pfc
+        merged_rows = set(prow.tolist()) | set(range(fc, pfc))
+        newzeros = 0
+        for c in range(fc, pfc):
+            stored = sum(1 for r in merged_rows if r >= c)
+            newzeros += stored - int(cc[c - 1])
+        assert newzeros == w * (int(cc[pfc - 1]) + w - int(cc[fc - 1])), "merge zeros are not the chain closed form"
+        merge = False
+        tot = zeros[g]
+        if (W <= rp.maxSize or rp.maxSize == 0) and rp.nrelax0 > 0:
+            if W <= rp.nrelax0:
+                merge = True
+            elif newzeros == 0:
+                merge = True
+            else:
+                # the reference's stored size of the merged supernode, in
+                # double arithmetic (it can reach 0 or below when the parent
+                # group's first column has fewer entries than the group's width)
+                size = np.float64(W * (W + 1) / 2 + W * (int(cc[pfc - 1]) - pw))
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    z = np.float64(tot + newzeros) / size
+                tot += newzeros
+                merge = (W <= rp.nrelax1 and z < rp.zrelax0) or (W <= rp.nrelax2 and z < rp.zrelax1) or z < rp.zrelax2
+        if merge:
+            ncols[k] += ncols[g]
+            zeros[k] = tot
+            group_start[g] = k
+    return [fund[k] for k in range(nf) if rep(k) == k] + [n + 1]
 
 
 # ------------------------------------------------------------ the matrices
@@ -296,6 +372,9 @@ def _pin(res, n, cp, ri, rp, initial_perm):
             ms = F + R2
             if xs[I] <= pc < xs[I + 1] and ms <= rp.nrelax0 and (maxs == 0 or ms <= maxs):
                 pytest.fail("merge of size %d <= nrelax0 not taken at column %d" % (ms, xs[I]))
+
+    # the relaxation's merge decisions replayed from the pattern
+    assert replay_relaxation(L, par, fund, rp) == list(xs), "relaxation decisions differ"
 
     # structure of every relaxed supernode = union of its columns' patterns
     for I in range(len(xs) - 1):
