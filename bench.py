@@ -63,7 +63,7 @@ def precision_leg(k, nrhs, reps):
         rp = sp.RelaxationParameters()
         rp.SetMaxSize(150)
         M = sp.symPACKMatrix2D()
-        M.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision=prec, ir_tol=1e-12))
+        M.Init(sp.symPACKOptions(ordering="NDBOX", relax=rp, precision=prec, ir_tol=1e-12, autotune=True))
         M.SymbolicFactorization(A)
         torch.cuda.synchronize()
         M.DistributeMatrixDevice(vals.data_ptr())
@@ -127,7 +127,7 @@ def ldl_leg(k, nrhs, reps):
         rp.SetMaxSize(150)
         M = sp.symPACKMatrix2D()
         M.Init(sp.symPACKOptions(ordering="NDBOX", decomposition="LDL", relax=rp, precision=prec,
-                                 ir_tol=1e-12))
+                                 ir_tol=1e-12, autotune=True))
         M.SymbolicFactorization(A)
         torch.cuda.synchronize()
         M.DistributeMatrixDevice(vals.data_ptr())
@@ -339,7 +339,7 @@ def main():
         rp = sp.RelaxationParameters()
         rp.SetMaxSize(150)
         M = sp.symPACKMatrix2D()
-        M.Init(sp.symPACKOptions(ordering="NDBOX", decomposition=fact, relax=rp))
+        M.Init(sp.symPACKOptions(ordering="NDBOX", decomposition=fact, relax=rp, autotune=True))
         M.SymbolicFactorization(A)
         M.DistributeMatrix(A)
         M.Factorize()
@@ -356,7 +356,7 @@ def main():
     n = A.size
     rp = sp.RelaxationParameters()
     rp.SetMaxSize(150)
-    opts = sp.symPACKOptions(ordering="NDBOX", decomposition="LL", relax=rp, precision="FP64")
+    opts = sp.symPACKOptions(ordering="NDBOX", decomposition="LL", relax=rp, precision="FP64", autotune=True)
     M = sp.symPACKMatrix2D()
     M.Init(opts)
     t0 = time.perf_counter()
@@ -510,7 +510,7 @@ def main():
         "config": {"workload": "3D 7-pt variable-coefficient diffusion %d^3 (n=%d), NDBOX ordering, "
                                "FP64 LL^T factor + solve nrhs=1" % (args.k, n),
                    "n": n, "nnz_lower": A.nnz, "ordering": "NDBOX", "decomposition": "LL",
-                   "relax": "8/32/64, max width 150", "nsuper": st["nsuper"],
+                   "relax": "8/32/64, max width 150", "autotune": True, "nsuper": st["nsuper"],
                    "etree_levels": st["nlevels"], "flops_sum_cc2": flops, "nnz_L": st["nnz_l"]},
         "factor_ms": round(fac_mean, 3),
         "factor_ms_min": round(float(np.min(fac_ms)), 3),

@@ -81,13 +81,14 @@ typedef struct spk_options {
                          and sweep; for LL^T and for the FP32 factor under refinement, not for
                          FP64 static LDL^T (explicit inverses of its indefinite diagonal blocks
                          lose accuracy).  0: block substitution, four launches per level */
-  int autotune;       /* 1 (default): the first spk_factorize of a structure with at least 1e9
+  int autotune;       /* 1: the first spk_factorize of a structure with at least 1e9
                          Schur-update flops times every etree level's Schur launch under each
                          kernel (64 x 64 tiles staged through LDS, 128 x 64 tiles, 64 x 64 tiles
                          with one wave per tile), task order (plan / longest first) and k split
                          (1-8 workgroups per tile) on the device and keeps the fastest
-                         (spk_get_autotune); 0: the plan's heuristic.  No reference
-                         counterpart */
+                         (spk_get_autotune), for structures factored many times (tuning costs
+                         tens of factorizations once); 0 (default): the plan's heuristic.  No
+                         reference counterpart */
   int schur_kernel;   /* 0 (default): the Schur-update kernel the autotuner (or, untuned, the
                          plan) chooses per level; 1: 64 x 64 tiles staged through LDS, 2: 128 x
                          64 tiles, 3: 64 x 64 tiles with one wave per tile, on every level (no

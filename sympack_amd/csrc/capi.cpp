@@ -145,7 +145,7 @@ void spk_default_options(spk_options* o) {
   o->solve_dataflow = 0;
   o->refine = 0;
   o->solve_inverse = 1;
-  o->autotune = 1;
+  o->autotune = 0;
   o->schur_kernel = 0;
 }
 

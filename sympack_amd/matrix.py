@@ -73,7 +73,7 @@ class symPACKOptions:
     def __init__(self, ordering="MMD", decomposition="LL", relax=None, perm=None,
                  precision="FP64", ir_max_iter=30, ir_tol=1e-12, verbose=0,
                  profile=False, device=0, merge_updates=1, solve_dataflow=False,
-                 order_refinement_str="NO", solve_inverse=True, autotune=True, schur_kernel=0):
+                 order_refinement_str="NO", solve_inverse=True, autotune=False, schur_kernel=0):
         self.orderingStr = ordering
         self.decomposition = decomposition
         self.relax = relax if relax is not None else RelaxationParameters()

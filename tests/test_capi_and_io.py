@@ -193,5 +193,8 @@ def test_schur_kernel_option_is_validated():
     for bad in (4, -1, 64):
         with pytest.raises(ValueError, match="schur_kernel"):
             m.Init(sp.symPACKOptions(schur_kernel=bad))
-    o = sp.symPACKOptions.from_args(["-schur_kernel", "2", "-autotune", "0"])
-    assert o.schur_kernel == 2 and o.autotune is False
+    o = sp.symPACKOptions.from_args(["-schur_kernel", "2", "-autotune", "1"])
+    assert o.schur_kernel == 2 and o.autotune is True
+    # tuning costs tens of factorizations: off unless asked for
+    assert sp.symPACKOptions().autotune is False
+    assert sp.symPACKOptions.from_args([]).autotune is False
