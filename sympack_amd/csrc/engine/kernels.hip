@@ -305,6 +305,93 @@ struct DiagLds {
   int store;  // fused: this workgroup writes the factored block back
 };
 
+// One block column J of a diagonal block held in LDS (tiles (I, J), I >= J,
+// of tl), factored lane per row by one wave: lanes 0-15 hold the 16 rows of
+// the diagonal tile (every wave that calls this keeps its own copy, so the
+// pivot columns are broadcast through the wave's LDS buffer cb without
+// barriers), lanes 16-63 hold 48 rows below it, starting 48 * wave rows
+// down.  On return a[] holds the lane's factored row (LL: L_ij; LDL: l_ij
+// below the diagonal, d_j on it) and the rows below are written back; the
+// diagonal tile is not (every wave reads it unfactored: the caller writes it
+// after a barrier through the returned row pointer).
+template <typename T, bool LDL>
+__device__ __forceinline__ T* chain_column(T* __restrict__ tl, T (*__restrict__ cb)[SUB], int J, int nb,
+                                           int wave, int lane, T (&a)[SUB]) {
+  const bool diag_lane = lane < SUB;
+  const int rb = 48 * wave + lane - SUB;  // row below the diagonal tile (lanes >= 16)
+  const int Ib = J + 1 + (diag_lane ? 0 : rb / SUB);
+  const bool row_ok = diag_lane || (Ib < nb);
+  const int I = diag_lane ? J : min(Ib, nb - 1);
+  const int rr = diag_lane ? lane : rb % SUB;
+  T* trow = tl + tslot(I, J) * TSZ + rr;
+#pragma unroll
+  for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
+  // every wave reads the diagonal tile: wave 0 writes the factored one
+  // back only after the barrier that ends this phase
+  // Column j of the diagonal rows (still unscaled) is broadcast through
+  // this wave's LDS buffer: LDS operations of one wave complete in order,
+  // so no barrier is needed between the write and the reads.  Column j+1
+  // is final on the diagonal rows as soon as step j has updated it (the
+  // first thing step j does), so it is published and read back in step j,
+  // a step ahead of its use: the LDS round trip stays off the wave's path.
+  T u[SUB];  // u[k], k >= j+3: column j of diagonal row k
+  if (diag_lane) cb[0][lane] = a[0];
+  compiler_fence();
+#pragma unroll
+  for (int k = 3; k < SUB; ++k) u[k] = cb[0][k];
+  compiler_fence();
+#pragma unroll
+  for (int j = 0; j < SUB; ++j) {
+    // The dependency chain from one pivot to the next runs through
+    // readlane(pivot) -> 1/pivot -> the next row's diagonal entry; the
+    // next row's entry comes by readlane, and the scaling 1/sqrt(pivot)
+    // stays off the chain.
+    const T piv = readlane_t(a[j], j);
+    // u[j+1] and u[j+2] feed the next two pivots: readlane
+    if (j + 1 < SUB) u[j + 1] = readlane_t(a[j], j + 1);
+    if (j + 2 < SUB) u[j + 2] = readlane_t(a[j], j + 2);
+    // 1/piv from the hardware estimate y0 and e = 1 - piv y0:
+    // inv = y0 (1 + s) with s = e + e^2 (f64; |e| < 2^-24, so the error is
+    // e^3) or s = e (f32)
+    const T y0 = rcp_est(piv);
+    const T e = fma(-piv, y0, T(1));
+    const T s = rcp_corr(e);
+    const T inv = fma(y0, s, y0);
+    // a_k -= (a_j / piv) u_k.  The entry that feeds the next pivot is
+    // a - t (1 + s) with t = a_j u_{j+1} y0: t and e are formed side by
+    // side, so it is three dependent operations after the estimate
+    if (j + 1 < SUB) {
+      const T t = (a[j] * u[j + 1]) * y0;
+      a[j + 1] = fma(-t, s, a[j + 1] - t);
+    }
+    // column j+1 of the diagonal rows is final: publish it, read the next
+    // step's entries (k >= j+4)
+    T un[SUB];
+    if (j + 1 < SUB) {
+      if (diag_lane) cb[(j + 1) & 1][lane] = a[j + 1];
+      compiler_fence();
+#pragma unroll
+      for (int k = j + 4; k < SUB; ++k) un[k] = cb[(j + 1) & 1][k];
+      compiler_fence();
+    }
+    const T l = a[j] * inv;
+#pragma unroll
+    for (int k = j + 2; k < SUB; ++k) a[k] -= l * u[k];
+    // LL: L_ij = a_ij / sqrt(d_j), with 1/sqrt(d_j) formed from the
+    // uniform pivot off the chain; LDL stores l = a_ij / d_j and d_j
+    if (LDL) a[j] = (lane == j) ? piv : l;
+    else a[j] = a[j] * rsqrt_nr(piv);
+#pragma unroll
+    for (int k = j + 4; k < SUB; ++k) u[k] = un[k];
+  }
+  // rows below: every valid lane (the trailing update reads them)
+  if (!diag_lane && row_ok) {
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) trow[c * TSTR] = a[c];
+  }
+  return trow;
+}
+
 // FUSED: the nt workgroups of a supernode (one per TRSM row tile) each
 // factor the block; the inverse tiles then replace the factored diagonal
 // tiles in LDS (and 1/D of every column is kept for LDL).  The factored block
@@ -379,82 +466,11 @@ __device__ __forceinline__ void diag_body(const Task4& t, const SnInfo& s, T* __
   }
 
   const bool diag_lane = lane < SUB;
-  const int rb = 48 * wave + lane - SUB;  // row below the diagonal tile (lanes >= 16)
   for (int J = 0; J < nb; ++J) {
     // ---- 1. block column J, lane per row (its tiles were written to LDS
     // by the initial load or at the end of step J-1) ----
-    const int Ib = J + 1 + (diag_lane ? 0 : rb / SUB);
-    const bool row_ok = diag_lane || (Ib < nb);
-    const int I = diag_lane ? J : min(Ib, nb - 1);
-    const int rr = diag_lane ? lane : rb % SUB;
-    T* trow = tl + tslot(I, J) * TSZ + rr;
     T a[SUB];
-#pragma unroll
-    for (int c = 0; c < SUB; ++c) a[c] = trow[c * TSTR];
-    // every wave reads the diagonal tile: wave 0 writes the factored one
-    // back only after the barrier that ends this phase
-    // Column j of the diagonal rows (still unscaled) is broadcast through
-    // this wave's LDS buffer: LDS operations of one wave complete in order,
-    // so no barrier is needed between the write and the reads.  Column j+1
-    // is final on the diagonal rows as soon as step j has updated it (the
-    // first thing step j does), so it is published and read back in step j,
-    // a step ahead of its use: the LDS round trip stays off the wave's path.
-    T (*__restrict__ cb)[SUB] = colb[wave];
-    T u[SUB];  // u[k], k >= j+3: column j of diagonal row k
-    if (diag_lane) cb[0][lane] = a[0];
-    compiler_fence();
-#pragma unroll
-    for (int k = 3; k < SUB; ++k) u[k] = cb[0][k];
-    compiler_fence();
-#pragma unroll
-    for (int j = 0; j < SUB; ++j) {
-      // The dependency chain from one pivot to the next runs through
-      // readlane(pivot) -> 1/pivot -> the next row's diagonal entry; the
-      // next row's entry comes by readlane, and the scaling 1/sqrt(pivot)
-      // stays off the chain.
-      const T piv = readlane_t(a[j], j);
-      // u[j+1] and u[j+2] feed the next two pivots: readlane
-      if (j + 1 < SUB) u[j + 1] = readlane_t(a[j], j + 1);
-      if (j + 2 < SUB) u[j + 2] = readlane_t(a[j], j + 2);
-      // 1/piv from the hardware estimate y0 and e = 1 - piv y0:
-      // inv = y0 (1 + s) with s = e + e^2 (f64; |e| < 2^-24, so the error is
-      // e^3) or s = e (f32)
-      const T y0 = rcp_est(piv);
-      const T e = fma(-piv, y0, T(1));
-      const T s = rcp_corr(e);
-      const T inv = fma(y0, s, y0);
-      // a_k -= (a_j / piv) u_k.  The entry that feeds the next pivot is
-      // a - t (1 + s) with t = a_j u_{j+1} y0: t and e are formed side by
-      // side, so it is three dependent operations after the estimate
-      if (j + 1 < SUB) {
-        const T t = (a[j] * u[j + 1]) * y0;
-        a[j + 1] = fma(-t, s, a[j + 1] - t);
-      }
-      // column j+1 of the diagonal rows is final: publish it, read the next
-      // step's entries (k >= j+4)
-      T un[SUB];
-      if (j + 1 < SUB) {
-        if (diag_lane) cb[(j + 1) & 1][lane] = a[j + 1];
-        compiler_fence();
-#pragma unroll
-        for (int k = j + 4; k < SUB; ++k) un[k] = cb[(j + 1) & 1][k];
-        compiler_fence();
-      }
-      const T l = a[j] * inv;
-#pragma unroll
-      for (int k = j + 2; k < SUB; ++k) a[k] -= l * u[k];
-      // LL: L_ij = a_ij / sqrt(d_j), with 1/sqrt(d_j) formed from the
-      // uniform pivot off the chain; LDL stores l = a_ij / d_j and d_j
-      if (LDL) a[j] = (lane == j) ? piv : l;
-      else a[j] = a[j] * rsqrt_nr(piv);
-#pragma unroll
-      for (int k = j + 4; k < SUB; ++k) u[k] = un[k];
-    }
-    // rows below: every valid lane (the trailing update reads them)
-    if (!diag_lane && row_ok) {
-#pragma unroll
-      for (int c = 0; c < SUB; ++c) trow[c * TSTR] = a[c];
-    }
+    T* trow = chain_column<T, LDL>(tl, colb[wave], J, nb, wave, lane, a);
     if (LDL && wave == 0 && diag_lane) dsh[lane] = a[lane];
     __syncthreads();
     // every wave has read the unfactored diagonal tile: wave 0 stores the
