@@ -4,7 +4,9 @@ level) and the rest, which P's own update computes as extra k columns.  That
 is only valid when P is I's etree parent and every row of struct(I) beyond
 P's columns lies in struct(P); the plan also keeps each merge to one child,
 never defers a supernode that absorbs one, and requires the deferred rows to
-cover most of P's off-diagonal rows."""
+cover most of P's off-diagonal rows.  The decisive check replays the merged
+updates on the dense pattern: every (source supernode, target entry) pair of
+the unmerged right-looking algorithm is computed exactly once."""
 import numpy as np
 import pytest
 
@@ -58,6 +60,31 @@ def test_merge_rules(name, gen, maxsize):
         prow = lx[xl[P] + wp:xl[P + 1]]
         assert np.all(np.isin(r2, prow)), "deferred rows lie in struct(P)"
         assert len(r2) >= 32 and len(r2) >= 0.85 * len(prow)
+    # replay: supernode X's tasks compute, for the pairs (r1 >= r2) of X's
+    # off-diagonal rows with r2 below the cut (a deferred X: its parent's
+    # columns), the contributions of X itself and of the child X absorbed
+    # (rows the child lacks contribute nothing)
+    W = np.diff(xs)
+    rows_of = [lx[xl[I] + W[I]:xl[I + 1]] for I in range(ns)]
+    child_of = {int(P): int(I) for I, P in enumerate(d) if P >= 0}
+    count = {}
+    for X in range(ns):
+        rows = rows_of[X]
+        if len(rows) == 0:
+            continue
+        cut_end = xs[d[X] + 1] if d[X] >= 0 else np.iinfo(np.int64).max
+        for D in [X] + ([child_of[X]] if X in child_of else []):
+            have = set(rows_of[D].tolist())
+            rr = [r for r in rows.tolist() if r in have]
+            for i, r2 in enumerate(rr):
+                if r2 >= cut_end:
+                    break
+                for r1 in rr[i:]:
+                    count[(D, r1, r2)] = count.get((D, r1, r2), 0) + 1
+    expected = {(D, r1, r2) for D in range(ns) for i, r2 in enumerate(rows_of[D].tolist())
+                for r1 in rows_of[D][i:].tolist()}
+    assert set(count) == expected, "every unmerged update entry is computed"
+    assert all(v == 1 for v in count.values()), "and only once"
     # off: nothing deferred
     assert np.all(_plan(gen, maxsize, merge=0).GetUpdateMerges() == -1)
 
